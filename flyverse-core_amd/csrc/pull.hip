@@ -1,0 +1,127 @@
+// Deterministic synaptic input for the plain (B, N) layout: no atomics, the reference CPU order.
+//
+// The reference's CPU path (flyverse/brain.py:679-692) adds w*x into every target in ascending presynaptic order
+// after the decay g *= a_s (brain.py:832), skipping silent presynaptic cells; its CUDA event_scatter
+// (flyverse/kernels/neural.cu:115-123) uses float atomicAdd and does not repeat run to run.  Here every target row
+// is owned by one wavefront, which reads its in-edges (post-major CSR, columns ascending), ballots the ones whose
+// presynaptic cell transmitted, and folds their products into the decayed g in lane (= presynaptic) order.
+#include "common.h"
+
+namespace {
+
+// One wavefront per (fly, post).  ptr/idx: post-major CSR (int32), columns ascending; w: weights (float or half).
+template <typename W>
+__global__ __launch_bounds__(256) void k_pull(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                              const W* __restrict__ w, const float* __restrict__ x,
+                                              float* __restrict__ g, int n, int batch, float a_s, int decay) {
+    int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (gw >= (int64_t)n * batch) return;
+    int lane = threadIdx.x & 63;
+    int b = (int)(gw / n), row = (int)(gw % n);
+    const float* xb = x + (int64_t)b * n;
+    float acc = g[gw];
+    if (decay) acc = acc * a_s;
+    int start = ptr[row], end = ptr[row + 1];
+    for (int base = start; base < end; base += kWave) {
+        int e = base + lane;
+        bool act = false;
+        float c = 0.0f;
+        if (e < end) {
+            float xv = xb[idx[e]];
+            if (xv != 0.0f) { act = true; c = (float)w[e] * xv; }
+        }
+        uint64_t m = __builtin_amdgcn_ballot_w64(act);
+        while (m) {
+            int j = __builtin_ctzll(m);
+            acc = acc + fh_readlane(c, j);
+            m &= m - 1;
+        }
+    }
+    if (lane == 0) g[gw] = acc;
+}
+
+// CSC-compatible deterministic scatter: the contract of launch_events (neural.cu:124) without atomics.  One
+// wavefront per (fly, tile of kTile posts) holds g's tile in LDS and walks the transmitting presynaptic cells in
+// ascending order; a cell's out-edges have distinct posts, so one pass over them needs no ordering, and the walk
+// itself gives every post its contributions in ascending presynaptic order.  Work grows with the number of tiles,
+// so this is the small-graph path; Brain uses the pull above.
+constexpr int kTile = 2048;
+
+template <typename W>
+__global__ __launch_bounds__(64) void k_events(const int* __restrict__ ptr, const int* __restrict__ post,
+                                               const W* __restrict__ w, const float* __restrict__ x,
+                                               float* __restrict__ g, const int* __restrict__ pre_ids, int n,
+                                               int count) {
+    __shared__ float tile[kTile];
+    int lane = threadIdx.x;
+    int b = blockIdx.y;
+    int lo = blockIdx.x * kTile, hi = min(lo + kTile, n);
+    float* gb = g + (int64_t)b * n;
+    const float* xb = x + (int64_t)b * n;
+    for (int k = lane; k < hi - lo; k += kWave) tile[k] = gb[lo + k];
+    __syncthreads();
+    int size = count < 0 ? n : count;
+    for (int base = 0; base < size; base += kWave) {
+        int k = base + lane;
+        int pre = 0;
+        float xv = 0.0f;
+        if (k < size) { pre = count < 0 ? k : pre_ids[k]; xv = xb[pre]; }
+        uint64_t m = __builtin_amdgcn_ballot_w64(xv != 0.0f);
+        while (m) {
+            int j = __builtin_ctzll(m);
+            m &= m - 1;
+            int p = __builtin_amdgcn_readlane(pre, j);
+            float xp = fh_readlane(xv, j);
+            int s = ptr[p], e = ptr[p + 1];
+            for (int q0 = s; q0 < e; q0 += kWave) {
+                int q = q0 + lane;
+                int t = q < e ? post[q] : n;
+                if (t >= lo && t < hi) {
+                    float c = (float)w[q] * xp;
+                    tile[t - lo] = tile[t - lo] + c;
+                }
+                // posts are ascending within the column: stop once this round has passed the tile
+                if (__builtin_amdgcn_readlane(t, kWave - 1) >= hi) break;
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = lane; k < hi - lo; k += kWave) gb[lo + k] = tile[k];
+}
+
+}  // namespace
+
+// g[b, post] = g[b, post] * a_s (if flags & 2 is clear) + sum over in-edges in ascending presynaptic order of
+// w * x[b, pre] for x != 0.  p = [ptr int32 (n+1), idx int32 (nnz), w (float, or half if flags & 1),
+// x float (B, n), g float (B, n)].
+FH_EXPORT int launch_pull(void** p, int n, int batch, float a_s, int flags, void* stream) {
+    if (n < 0 || batch < 0) return FH_ERR_ARGS;
+    int64_t waves = (int64_t)n * batch;
+    if (waves == 0) return 0;
+    if (waves * 64 >= (1ll << 40)) return FH_ERR_SIZE;
+    unsigned blocks = (unsigned)((waves + 3) / 4);
+    int decay = (flags & 2) ? 0 : 1;
+    if (flags & 1)
+        hipLaunchKernelGGL(k_pull<__half>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
+                           (const int*)p[1], (const __half*)p[2], (const float*)p[3], (float*)p[4], n, batch, a_s, decay);
+    else
+        hipLaunchKernelGGL(k_pull<float>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
+                           (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], n, batch, a_s, decay);
+    FH_LAUNCH_CHECK();
+}
+
+// Same contract as flyverse/kernels/neural.cu:124 (launch_events): p = [ptr, post, w, x, g, pre_ids] with the
+// pre-major CSC of Brain.set_weights (flyverse/brain.py:530-540); count < 0 walks every presynaptic cell, else the
+// `count` ascending entries of pre_ids.  Deterministic: the result equals the reference CPU order bit for bit.
+FH_EXPORT int launch_events(void** p, int n, int batch, int half, int count, void* stream) {
+    if (n < 0 || batch < 0) return FH_ERR_ARGS;
+    if ((int64_t)n * batch == 0) return 0;
+    dim3 grid((unsigned)((n + kTile - 1) / kTile), (unsigned)batch);
+    if (half)
+        hipLaunchKernelGGL(k_events<__half>, grid, dim3(64), 0, (hipStream_t)stream, (const int*)p[0], (const int*)p[1],
+                           (const __half*)p[2], (const float*)p[3], (float*)p[4], (const int*)p[5], n, count);
+    else
+        hipLaunchKernelGGL(k_events<float>, grid, dim3(64), 0, (hipStream_t)stream, (const int*)p[0], (const int*)p[1],
+                           (const float*)p[2], (const float*)p[3], (float*)p[4], (const int*)p[5], n, count);
+    FH_LAUNCH_CHECK();
+}

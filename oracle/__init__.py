@@ -1,0 +1,323 @@
+"""CPU oracle for the brain step — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this package, and only as the checker
+(or, in bench.py, as the timed CPU baseline); the product path (the HIP library and the flyhip package) never does.
+
+It wraps oracle/oracle.c (a plain-C restatement of the reference arithmetic, built by oracle/Makefile) and composes
+the reference's step sequences from it:
+  * `brain_step`   — Brain._step_inference with the event-driven CPU synaptic input (flyverse/brain.py:818-870,
+                      679-692), the Poisson draw replaced by the Philox uniforms of `uniform`;
+  * `optic_frame`  — OpticLobe._step_frame_inference (flyverse/optic.py:563-604) with the substep of :525-551;
+  * `fly_run`      — the FlyBrain frame loop (flyverse/fly.py:477-507): optic frame first, its drive copied into
+                      the LIF, then the LIF steps of the frame.
+
+Pinning (what checks each function against something other than itself), by function:
+  philox4x32_10  Random123 known-answer vectors (tests/test_oracle.py).
+  lif            closed forms of the exponential-Euler membrane, the fp32 refractory chain 2.2f - k*0.1f, the
+                 threshold equality case, frozen/forced/STD/adaptation branches (tests/test_oracle.py), and a torch
+                 float32 restatement of flyverse/brain.py:838-868 run op by op (tests/test_oracle.py).
+  events         brute-force dense float64 x @ W.T within 1e-6 relative; bit-exact against an explicit
+                 Python loop over nonzero(x) in (b, pre) order on small graphs (tests/test_oracle.py).
+  spmv           scipy float64 M @ x within 1e-5 relative; the chunk order against a pure-Python fold
+                 (tests/test_oracle.py); half rounding against numpy float16 casting.
+  optic_dr/update  the torch float32 expressions of flyverse/optic.py:528-551 evaluated op by op (exact).
+  photoreceptor  the torch float32 expressions of flyverse/optic.py:505-513 evaluated op by op (exact).
+  group_means    numpy float64 means.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_I, _F, _U64, _U32, _L = ct.c_int, ct.c_float, ct.c_uint64, ct.c_uint32, ct.c_int64
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ct.CDLL(str(LIB_PATH))
+        L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_uniform1.argtypes = [_U64, _U32, _U32, _U32, _U32]
+        L.orc_uniform1.restype = ct.c_float
+        L.orc_uniform.argtypes = [_f32p, _I, _I, _U64, _U32, _U32]
+        L.orc_lif.argtypes = [_I, _I, _I] + [_f32p] * 16
+        L.orc_scale.argtypes = [_f32p, _L, _F]
+        L.orc_events.argtypes = [_I, _I, _i64p, _i32p, _f32p, _f32p, _f32p]
+        L.orc_spmv.argtypes = [_I, _I, _I, _i64p, _i32p, _f32p, _f32p, _f32p, _I]
+        L.orc_optic_dr.argtypes = [_I, _I, _f32p, _f32p, _f32p]
+        L.orc_optic_update.argtypes = [_I, _I] + [_f32p] * 8 + [_F, _F, _F]
+        L.orc_photoreceptor.argtypes = [_I, _I, _f32p, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p]
+        L.orc_group_means.argtypes = [_I, _I, _I, _I, _i32p, _i32p, _f32p, _f64p]
+        L.orc_drive_out.argtypes = [_I, _I, _I, _f32p, _i32p, _F, _F, _f32p]
+        L.orc_feedback.argtypes = [_I, _I, _I, _f32p, _i32p, _f32p]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ------------------------------------------------------------------ primitives
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def uniform(batch, n, seed, step, stream=0):
+    out = np.empty((batch, n), np.float32)
+    lib().orc_uniform(out, batch, n, int(seed) & (2**64 - 1), int(step) & 0xFFFFFFFF, stream)
+    return out
+
+
+def uniform1(seed, step, neuron, fly=0, stream=0) -> float:
+    return float(lib().orc_uniform1(int(seed) & (2**64 - 1), step & 0xFFFFFFFF, neuron, fly, stream))
+
+
+def lif(state: dict, flags: int, P, rnd=None, phase=None):
+    """In-place fused LIF update on a dict of float32 arrays (keys as Brain's tensors; `out` = the spike_buf slot)."""
+    s = state
+    B, N = s["v"].shape
+    P = _f32(P)
+    rnd = s["poisson_p"] if rnd is None else _f32(rnd)
+    phase = P if phase is None else _f32(phase)
+    lib().orc_lif(N, B, flags, s["v"], s["g"], s["drive"], s["adapt"], s["refrac"], s["active"], s["poisson_p"], rnd,
+                  s["res"], s["std_u"], s["spikes"], s["out"], s["rate"], s["spike_counts"], P, phase)
+
+
+def scale(x, a):
+    lib().orc_scale(x.reshape(-1), x.size, np.float32(a))
+
+
+def events(csc_ptr, csc_post, csc_w, x, g):
+    B, N = x.shape
+    lib().orc_events(N, B, np.ascontiguousarray(csc_ptr, np.int64), np.ascontiguousarray(csc_post, np.int32),
+                     _f32(csc_w), _f32(x), g)
+
+
+def spmv(ptr, idx, w, x, half=False):
+    rows = len(ptr) - 1
+    B, cols = x.shape
+    y = np.empty((B, rows), np.float32)
+    lib().orc_spmv(rows, cols, B, np.ascontiguousarray(ptr, np.int64), np.ascontiguousarray(idx, np.int32), _f32(w),
+                   _f32(x), y, int(half))
+    return y
+
+
+def optic_dr(v, b):
+    B, n = v.shape
+    dr = np.empty_like(v)
+    lib().orc_optic_dr(n, B, _f32(v), _f32(b), dr)
+    return dr
+
+
+def optic_update(v, adapt, dr, y, pr, spk, a, b, gain_rr, adapt_gain, a_ad):
+    B, n = v.shape
+    lib().orc_optic_update(n, B, v, adapt, dr, _f32(y), _f32(pr), _f32(spk), _f32(a), _f32(b), np.float32(gain_rr),
+                           np.float32(adapt_gain), np.float32(a_ad))
+
+
+def photoreceptor(rad, sens, I_lp, I_mean, fresh, coefs):
+    B, n_pr = I_lp.shape
+    a_pr = np.empty_like(I_lp)
+    lib().orc_photoreceptor(n_pr, B, _f32(rad), _f32(sens), I_lp, I_mean, fresh, _f32(coefs), a_pr)
+    return a_pr
+
+
+def group_means(ptr, idx, rates, stride=None):
+    B, n = rates.shape
+    groups = len(ptr) - 1
+    stride = groups if stride is None else stride
+    out = np.zeros((B, stride), np.float64)
+    lib().orc_group_means(n, groups, B, stride, np.ascontiguousarray(ptr, np.int32), np.ascontiguousarray(idx, np.int32),
+                          _f32(rates), out)
+    return out
+
+
+def drive_out(y_sr, spk_idx, n, gain_out, clip):
+    B = y_sr.shape[0]
+    drive = np.empty((B, n), np.float32)
+    lib().orc_drive_out(n, len(spk_idx), B, _f32(y_sr), np.ascontiguousarray(spk_idx, np.int32), np.float32(gain_out),
+                        np.float32(clip), drive)
+    return drive
+
+
+def feedback(rate, spk_idx):
+    B, n = rate.shape
+    s = np.empty((B, len(spk_idx)), np.float32)
+    lib().orc_feedback(n, len(spk_idx), B, _f32(rate), np.ascontiguousarray(spk_idx, np.int32), s)
+    return s
+
+
+# ------------------------------------------------------------------ composite reference sequences
+def lif_params_vector(p) -> np.ndarray:
+    """[v_rest, v_reset, v_th, a_m, dt, t_ref, adapt_jump, a_ad, a_std, a_r, rate_gain] (flyverse/brain.py:493-520),
+    coefficients in host double, then float32."""
+    import math
+    a_m = math.exp(-p.dt / p.tau_m)
+    a_r = math.exp(-p.dt / p.rate_tau)
+    a_ad = math.exp(-p.dt / p.adapt_tau) if p.adapt_jump > 0 else 0.0
+    a_std = math.exp(-p.dt / p.std_tau) if p.std_u > 0 else 1.0
+    return np.array([p.v_rest, p.v_reset, p.v_th, a_m, p.dt, p.t_ref, p.adapt_jump, a_ad, a_std, a_r,
+                     (1 - a_r) * 1000.0 / p.dt], dtype=np.float32)
+
+
+class BrainState:
+    """The (B, N) state of flyverse/brain.py:464-492, as float32 numpy arrays, plus the spike delay ring."""
+    FIELDS = ("v", "g", "refrac", "drive", "poisson_p", "rate", "spikes", "spike_counts", "adapt", "res")
+
+    def __init__(self, B, N, n_delay, v_rest):
+        self.v = np.full((B, N), v_rest, np.float32)
+        for k in ("g", "refrac", "drive", "poisson_p", "rate", "spikes", "spike_counts", "adapt"):
+            setattr(self, k, np.zeros((B, N), np.float32))
+        self.res = np.ones((B, N), np.float32)
+        self.active = np.ones(N, np.float32)
+        self.std_u = np.zeros(N, np.float32)
+        self.spike_buf = np.zeros((n_delay, B, N), np.float32)
+        self.buf_pos = 0
+        self.step_count = 0
+        self.n_delay = n_delay
+
+
+def brain_step(st: BrainState, csc, P, a_s, seed, flags, poisson_on=True, record=True):
+    """One Brain._step_inference step on the CPU (flyverse/brain.py:830-870): g *= a_s; the event-driven input of
+    the spikes emitted n_delay steps ago; the Poisson uniforms of step `step_count` (Philox, fly index = row); the
+    fused update; the ring advances.  csc = (ptr int64, post int32, w float32), pre-major."""
+    scale(st.g, a_s)
+    events(csc[0], csc[1], csc[2], st.spike_buf[st.buf_pos], st.g)
+    B, N = st.v.shape
+    fl = flags | (1 if poisson_on else 0) | (4 if record else 0)
+    rnd = uniform(B, N, seed, st.step_count) if poisson_on else st.poisson_p
+    slot = st.spike_buf[st.buf_pos]
+    d = dict(v=st.v, g=st.g, drive=st.drive, adapt=st.adapt, refrac=st.refrac, active=st.active,
+             poisson_p=st.poisson_p, res=st.res, std_u=st.std_u, spikes=st.spikes, out=slot, rate=st.rate,
+             spike_counts=st.spike_counts)
+    lif(d, fl, P, rnd)
+    st.buf_pos = (st.buf_pos + 1) % st.n_delay
+    st.step_count += 1
+
+
+class OpticState:
+    def __init__(self, B, n_rate, n_pr):
+        self.v = np.zeros((B, n_rate), np.float32)
+        self.adapt = np.zeros((B, n_rate), np.float32)
+        self.dr = np.zeros((B, n_rate), np.float32)
+        self.I_lp = np.zeros((B, n_pr), np.float32)
+        self.I_mean = np.zeros((B, n_pr), np.float32)
+        self.fresh = np.ones(B, np.uint8)
+        self.pending_ms = 0.0
+
+
+def optic_frame(os_: OpticState, g, rad, rate, frame_ms):
+    """OpticLobe._step_frame_inference (flyverse/optic.py:563-604) on graph `g` (a flyhip.synth.FlyGraph): returns the
+    (B, N) drive written into the LIF (fly.py:492)."""
+    a_pr = photoreceptor(rad, g.pr_sens, os_.I_lp, os_.I_mean, os_.fresh, g.pr_coefs(frame_ms))
+    s = feedback(rate, g.spk_idx)
+    total = os_.pending_ms + frame_ms
+    steps = int(np.floor((total + 1e-9) / g.optic.dt_ms))
+    os_.pending_ms = max(0.0, total - steps * g.optic.dt_ms)
+    B = rate.shape[0]
+    if steps:
+        pr_in = spmv(*g.W_rp, a_pr)
+        pr_in = (np.float32(g.optic.gain_in) * pr_in).astype(np.float32)
+        spk_in = spmv(*g.W_rs, s)
+        spk_in = (np.float32(g.optic.gain_fb) * spk_in).astype(np.float32)
+        dr = optic_dr(os_.v, g.b_vec)
+        for _ in range(steps):
+            y = spmv(*g.W_rr, dr)
+            optic_update(os_.v, os_.adapt, dr, y, pr_in, spk_in, g.a_vec, g.b_vec, g.optic.gain_rr,
+                         g.optic.adapt_gain, g.a_ad)
+        os_.dr = dr
+    dr = optic_dr(os_.v, g.b_vec)               # delta_rate = rates() - r0 with r0 = b (optic.py:599)
+    y_sr = spmv(*g.W_sr, dr)
+    return drive_out(y_sr, g.spk_idx, g.n, g.optic.gain_out_mv, g.optic.drive_clip_mv)
+
+
+# ------------------------------------------------------------------ the whole fly (flyverse/fly.py:448-507)
+class OptSpec:
+    """Host-built optic-lobe inputs of one graph (the arrays flyhip.optic.optic_matrices produces)."""
+    def __init__(self, m: dict, params, sens, n):
+        to = lambda M: (M.indptr.astype(np.int64), M.indices.astype(np.int32), M.data.astype(np.float32))
+        self.W_rr, self.W_rp, self.W_rs, self.W_sr = (to(m[k]) for k in ("M_rr", "M_rp", "M_rs", "M_sr"))
+        self.a_vec, self.b_vec, self.a_ad = m["a"], m["b"], m["a_ad"]
+        self.spk_idx = m["spk_idx"].astype(np.int32)
+        self.pr_sens = np.ascontiguousarray(sens, np.float32)
+        self.optic = params
+        self.n = n
+
+    def pr_coefs(self, frame_ms):
+        a_lp = float(np.exp(-frame_ms / self.optic.tau_lp_ms))
+        a_ad = float(np.exp(-frame_ms / self.optic.tau_adapt_ms))
+        return np.array([a_lp, 1 - a_lp, a_ad, 1 - a_ad, self.optic.eps, self.optic.contrast_clip], np.float32)
+
+
+def _per_fly_uniform(B, n, seeds, step, stream):
+    """seeds an int: one stream, fly index = row; a sequence: row b uses seeds[b] with fly index 0."""
+    if np.isscalar(seeds):
+        return uniform(B, n, int(seeds), step, stream)
+    return np.stack([uniform(1, n, int(s), step, stream)[0] for s in seeds])
+
+
+class FlyModel:
+    """B flies on the CPU: LIF (event-driven reference order) plus, optionally, the optic lobe coupled per frame."""
+    def __init__(self, n, B, P, a_s, n_delay, csc, active, seeds, optic: OptSpec | None = None, movie_seeds=None,
+                 v_rest=-52.0, dt=0.1):
+        self.st = BrainState(B, n, n_delay, v_rest)
+        self.st.active = np.ascontiguousarray(active, np.float32)
+        self.P, self.a_s, self.csc, self.seeds, self.dt = _f32(P), float(a_s), csc, seeds, dt
+        self.optic = optic
+        self.os = OpticState(B, len(optic.b_vec), len(optic.pr_sens)) if optic is not None else None
+        self.movie_seeds = movie_seeds
+        self.frame_index = 0
+        self.B, self.n = B, n
+
+    flags = 32                                    # frozen-at-rest skip, as Brain(cuda_compact=True)
+
+    def lif_steps(self, steps, poisson_on=None, record=True):
+        st = self.st
+        on = bool((st.poisson_p > 0).any()) if poisson_on is None else poisson_on
+        B, N = st.v.shape
+        for _ in range(steps):
+            scale(st.g, self.a_s)
+            events(*self.csc, st.spike_buf[st.buf_pos], st.g)
+            rnd = _per_fly_uniform(B, N, self.seeds, st.step_count, 0) if on else st.poisson_p
+            slot = st.spike_buf[st.buf_pos]
+            d = dict(v=st.v, g=st.g, drive=st.drive, adapt=st.adapt, refrac=st.refrac, active=st.active,
+                     poisson_p=st.poisson_p, res=st.res, std_u=st.std_u, spikes=st.spikes, out=slot, rate=st.rate,
+                     spike_counts=st.spike_counts)
+            lif(d, self.flags | (1 if on else 0) | (4 if record else 0), self.P, rnd)
+            st.buf_pos = (st.buf_pos + 1) % st.n_delay
+            st.step_count += 1
+            yield st
+
+    def frame(self, steps, raster=None):
+        """One frame: optic first (if any) with a fresh random movie, then `steps` LIF steps; appends spikes."""
+        if self.optic is not None:
+            npr = len(self.optic.pr_sens)
+            rad = _per_fly_uniform(self.B, npr * 4, self.movie_seeds, self.frame_index, 1).reshape(self.B, npr, 4)
+            self.st.drive[:] = optic_frame(self.os, self.optic, rad, self.st.rate, steps * self.dt)
+        for st in self.lif_steps(steps):
+            if raster is not None:
+                raster.append(st.spikes.copy())
+        self.frame_index += 1

@@ -1,0 +1,55 @@
+"""Builders shared by the tests: small seeded graphs and the oracle model of a flyhip FlyBrain."""
+import math
+
+import numpy as np
+import scipy.sparse as sp
+
+import oracle
+from flyhip import synth
+from flyhip.brain import LIFParams, shaped_weights
+from flyhip.optic import OpticParams, optic_matrices
+
+
+def random_csr(rng, rows, cols, density, long_rows=(), long_len=0, signed=True):
+    M = sp.random(rows, cols, density=density, random_state=rng, format="csr", dtype=np.float32)
+    if signed:
+        M.data -= 0.5
+    if long_rows:
+        M = M.tolil()
+        for r in long_rows:
+            cs = rng.choice(cols, min(long_len, cols), replace=False)
+            for c_ in cs:
+                M[r, c_] = np.float32(rng.standard_normal())
+        M = M.tocsr()
+    M.sum_duplicates()
+    M.sort_indices()
+    return M
+
+
+def fly_setup(scale=0.02, seed=1):
+    conn, retina, layout = synth.fly_graph(seed, scale)
+    lp = LIFParams.shiu(dt=0.1)
+    op = OpticParams(tau_by_type=conn.rate_tau, baseline_by_type=conn.rate_baseline, pair_gain=[])
+    return conn, retina, layout, lp, op
+
+
+def oracle_fly(conn, retina, lp, op, B, seeds, movie_seeds, with_optic=True):
+    """The CPU model of FlyBrain(conn, retina, lif_params=lp, optic_params=op) with the same host-built weights."""
+    W = shaped_weights(conn, lp)
+    active = np.ones(conn.n, np.float32)
+    m = optic_matrices(conn, retina, op) if with_optic else None
+    if m is not None:
+        keep = np.ones(conn.n, np.float32)
+        keep[m["rate_idx"]] = 0.0
+        active[m["rate_idx"]] = 0.0
+        W = (sp.diags(keep) @ W @ sp.diags(keep)).tocsr()
+        W.eliminate_zeros()
+    Wc = sp.csc_matrix(W)
+    Wc.sum_duplicates()
+    Wc.sort_indices()
+    csc = (Wc.indptr.astype(np.int64), Wc.indices.astype(np.int32), Wc.data.astype(np.float32))
+    P = oracle.lif_params_vector(lp)
+    n_delay = max(1, int(round(lp.delay / lp.dt)))
+    spec = oracle.OptSpec(m, op, retina.pr_sens, conn.n) if m is not None else None
+    return oracle.FlyModel(conn.n, B, P, math.exp(-lp.dt / lp.tau_syn), n_delay, csc, active, seeds, spec,
+                           movie_seeds, v_rest=lp.v_rest, dt=lp.dt)

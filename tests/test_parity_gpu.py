@@ -1,0 +1,322 @@
+"""GPU parity of every plain-path kernel against the CPU oracle, through the C ABI (flyhip.hip wraps libflyhip.so).
+
+The bar: integer, index and spike work bit-exact; the LIF, pull, optic and CSR kernels bit-exact too, because the
+oracle restates the same fp32 operation order (the reference CPU order for the LIF and the synaptic input, the
+defined chunked order for CSR products); CSR products additionally within 1e-5 relative of float64 scipy."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle
+from flyhip import hip, synth
+from flyhip.brain import Brain, LIFParams
+from flyhip.fly import FlyBrain
+from helpers import fly_setup, oracle_fly, random_csr
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def T(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device=DEV)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def assert_bits(a, b, what=""):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    if a.dtype.kind == "f":
+        ok = (a == b) | (np.isnan(a) & np.isnan(b))
+        if not ok.all():
+            i = np.argwhere(~ok)[0]
+            raise AssertionError(f"{what}: {(~ok).sum()} of {a.size} differ, first at {tuple(i)}: {a[tuple(i)]!r} vs {b[tuple(i)]!r}")
+    else:
+        assert np.array_equal(a, b), what
+
+
+class FakeBrain:
+    """The attributes lif_update reads, over arbitrary (B, N) state."""
+    def __init__(self, st, flags, P, B, N):
+        self.v, self.g, self.drive, self.adapt, self.refrac = (T(st[k]) for k in ("v", "g", "drive", "adapt", "refrac"))
+        self.active, self.poisson_p, self.res, self.std_u_vec = (T(st[k]) for k in ("active", "poisson_p", "res", "std_u"))
+        self.spikes, self.rate, self.spike_counts = T(st["spikes"]), T(st["rate"]), T(st["spike_counts"])
+        self.spike_buf = T(st["out"][None])
+        self.buf_pos = 0
+        self._hip_P = T(P)
+        self._poisson_on, self._std_on = bool(flags & 1), bool(flags & 2)
+        self.record_activity = bool(flags & 4)
+        self.cuda_compact = bool(flags & 32)
+        self.p = LIFParams(adapt_jump=float(P[6]) if flags & 8 else 0.0)
+        self.B, self.n = B, N
+
+
+def random_lif_state(rng, B, N):
+    st = {
+        "v": rng.uniform(-60, -40, (B, N)), "g": rng.normal(0, 5, (B, N)), "drive": rng.normal(0, 3, (B, N)),
+        "adapt": rng.uniform(0, 2, (B, N)), "refrac": rng.choice([0.0, 0.0, 0.1, 1.3, 2.2], (B, N)),
+        "active": (rng.random(N) > 0.2), "poisson_p": np.where(rng.random((B, N)) < 0.3, rng.uniform(0, 0.5, (B, N)), 0),
+        "res": rng.uniform(0.2, 1, (B, N)), "std_u": rng.uniform(0, 0.5, N), "spikes": np.zeros((B, N)),
+        "out": np.zeros((B, N)), "rate": rng.uniform(0, 50, (B, N)), "spike_counts": rng.integers(0, 5, (B, N)),
+    }
+    st = {k: np.ascontiguousarray(v, np.float32) for k, v in st.items()}
+    # frozen cells exactly at rest exercise the compact skip
+    rest = rng.random((B, N)) < 0.2
+    for k, val in (("v", -52.0), ("g", 0), ("drive", 0), ("adapt", 0), ("refrac", 0), ("rate", 0), ("res", 1)):
+        st[k][rest] = val
+    st["v"][0, :3] = -45.0                     # exactly at threshold
+    return st
+
+
+@pytest.mark.parametrize("B,flags", [(1, 0), (3, 1 | 4), (2, 1 | 2 | 4 | 8), (3, 1 | 32), (2, 1 | 2 | 8 | 32)])
+def test_lif_update_bit_exact(B, flags):
+    rng = np.random.default_rng(7 + flags)
+    N = 259
+    P = np.array([-52.0, -52.0, -45.0, math.exp(-0.1 / 20), 0.1, 2.2, 1.37, math.exp(-0.1 / 200),
+                  math.exp(-0.1 / 300), math.exp(-0.1 / 100), (1 - math.exp(-0.1 / 100)) * 1000 / 0.1], np.float32)
+    st = random_lif_state(rng, B, N)
+    rnd = rng.random((B, N)).astype(np.float32)
+    fb = FakeBrain(st, flags, P, B, N)
+    for it in range(5):
+        hip.lif_update(fb, T(rnd))
+        oracle.lif(st, flags, P, rnd)
+        torch.cuda.synchronize()
+        for k, t in (("v", fb.v), ("g", fb.g), ("adapt", fb.adapt), ("refrac", fb.refrac), ("res", fb.res),
+                     ("spikes", fb.spikes), ("rate", fb.rate), ("spike_counts", fb.spike_counts)):
+            assert_bits(N_(t), st[k], f"{k} iter {it}")
+        assert_bits(N_(fb.spike_buf[0]), st["out"], "out")
+        rnd = rng.random((B, N)).astype(np.float32)
+
+
+def N_(t):
+    return t.detach().cpu().numpy()
+
+
+def test_lif_phase_coefficients_bit_exact():
+    rng = np.random.default_rng(3)
+    B, N = 2, 300
+    P = np.array([-52.0, -52.0, -45.0, 0.99, 0.1, 2.2, 1.0, 0.995, 0.997, 0.999, 10.0], np.float32)
+    phase = np.stack([rng.uniform(0.9, 1, N), rng.choice([0.1, 0.2], N), rng.uniform(0.9, 1, N), rng.uniform(0.9, 1, N),
+                      rng.uniform(0.99, 1, N), rng.uniform(5, 10, N), rng.choice([0.0, 1.0], N), rng.choice([1.0, 2.0], N)]
+                     ).astype(np.float32)
+    st = random_lif_state(rng, B, N)
+    rnd = rng.random((B, N)).astype(np.float32)
+    flags = 1 | 2 | 4 | 8 | 16
+    fb = FakeBrain(st, flags, P, B, N)
+    hip.lif_update(fb, T(rnd), T(phase))
+    oracle.lif(st, flags, P, rnd, phase)
+    for k, t in (("v", fb.v), ("refrac", fb.refrac), ("spikes", fb.spikes), ("rate", fb.rate), ("res", fb.res)):
+        assert_bits(N_(t), st[k], k)
+
+
+def test_uniform_matches_oracle():
+    out = torch.empty(3, 1000, device=DEV)
+    for step in (0, 1, 12345, 2**32 - 1):
+        hip.uniform(out, 0xDEADBEEF12345678, step, stream_tag=1)
+        assert_bits(N_(out), oracle.uniform(3, 1000, 0xDEADBEEF12345678, step, 1), f"step {step}")
+    base = torch.tensor([77], dtype=torch.int32, device=DEV)
+    hip.uniform(out, 5, 3, stream_tag=0, step_base=base)
+    assert_bits(N_(out), oracle.uniform(3, 1000, 5, 80, 0), "step_base")
+
+
+def _csc(W):
+    Wc = sp.csc_matrix(W)
+    Wc.sum_duplicates()
+    Wc.sort_indices()
+    return Wc
+
+
+@pytest.mark.parametrize("B,density,activity", [(1, 0.05, 0.05), (3, 0.02, 0.3), (2, 0.2, 1.0), (1, 0.01, 0.0)])
+def test_pull_and_events_bit_exact(B, density, activity):
+    rng = np.random.default_rng(11)
+    n = 1500
+    W = random_csr(rng, n, n, density, long_rows=(3, 700), long_len=900)
+    W.data *= 7.0
+    x = (rng.random((B, n)) < activity).astype(np.float32)
+    x[:, ::7] *= rng.uniform(0.2, 1.0, (B, len(range(0, n, 7)))).astype(np.float32)   # STD-style fractional x
+    g0 = rng.normal(0, 4, (B, n)).astype(np.float32)
+    a_s = math.exp(-0.1 / 5)
+    Wc = _csc(W)
+    ref = g0.copy()
+    oracle.scale(ref, a_s)
+    oracle.events(Wc.indptr, Wc.indices, Wc.data, x, ref)
+    g = T(g0)
+    hip.pull(T(W.indptr.astype(np.int32)), T(W.indices.astype(np.int32)), T(W.data), T(x), g, a_s)
+    assert_bits(N_(g), ref, "pull")
+    # the CSC-compatible scatter (no decay), full and compact pre lists
+    for compact in (False, True):
+        g = T(g0)
+        hip.scale_(g, a_s)
+        pre_ids = T(np.flatnonzero(np.diff(Wc.indptr)).astype(np.int32)) if compact else None
+        hip.event_scatter(T(Wc.indptr.astype(np.int32)), T(Wc.indices.astype(np.int32)), T(Wc.data), T(x), g, pre_ids)
+        assert_bits(N_(g), ref, f"events compact={compact}")
+
+
+def test_pull_half_weights():
+    rng = np.random.default_rng(2)
+    n, B = 600, 2
+    W = random_csr(rng, n, n, 0.05)
+    wh = W.data.astype(np.float16)
+    x = (rng.random((B, n)) < 0.2).astype(np.float32)
+    g0 = rng.normal(0, 1, (B, n)).astype(np.float32)
+    ref = g0.copy()
+    oracle.scale(ref, 0.9)
+    Wc = _csc(sp.csr_matrix((wh.astype(np.float32), W.indices, W.indptr), shape=W.shape))
+    oracle.events(Wc.indptr, Wc.indices, Wc.data, x, ref)
+    g = T(g0)
+    hip.pull(T(W.indptr.astype(np.int32)), T(W.indices.astype(np.int32)), T(wh), T(x), g, 0.9)
+    assert_bits(N_(g), ref, "pull half")
+
+
+@pytest.mark.parametrize("B", [1, 3, 8])
+def test_csr_products_bit_exact_and_close_to_float64(B):
+    rng = np.random.default_rng(5 + B)
+    rows, cols = 2000, 1700
+    M = random_csr(rng, rows, cols, 0.03, long_rows=(0, 999, 1998), long_len=1650)
+    M = sp.vstack([M, sp.csr_matrix((3, cols), dtype=np.float32)]).tocsr()     # empty rows at the end
+    x = rng.standard_normal((B, cols)).astype(np.float32)
+    x[:, ::3] = 0
+    ref = oracle.spmv(M.indptr, M.indices, M.data, x)
+    csr = hip.CSR(M, DEV)
+    y = csr.matvec(T(x))
+    assert_bits(N_(y), ref, "chunked")
+    assert_bits(N_(csr.matvec_reference_shape(T(x))), ref, "launch_csr")
+    exact = (M.astype(np.float64) @ x.astype(np.float64).T).T
+    np.testing.assert_allclose(N_(y), exact, rtol=1e-5, atol=1e-5 * np.abs(exact).max())
+    y2 = csr.matvec(T(x), scale=3.0)
+    assert_bits(N_(y2), (np.float32(3.0) * ref).astype(np.float32), "scaled")
+    # float16 weights: x rounded to half first (the reference warp kernel's convention)
+    h = hip.CSR(M, DEV, torch.float16)
+    wh = M.data.astype(np.float16).astype(np.float32)
+    assert_bits(N_(h.matvec(T(x))), oracle.spmv(M.indptr, M.indices, wh, x, half=True), "half")
+
+
+def test_csr_empty_and_validation():
+    csr = hip.CSR(sp.csr_matrix((3, 4), dtype=np.float32), DEV)
+    x = torch.zeros(2, 4, device=DEV)
+    assert torch.equal(csr.matvec(x), torch.zeros(2, 3, device=DEV))
+    b = torch.zeros(4, device=DEV)
+    with pytest.raises(ValueError):
+        hip.optic_dr(x.double(), b, x)
+    with pytest.raises(ValueError):
+        hip.optic_dr(x[:, ::2], b[:2], x[:, ::2])
+    with pytest.raises(ValueError):
+        hip.optic_dr(x, b[:2], x)
+    v = torch.empty(2, 0, device=DEV)
+    hip.optic_dr(v, b[:0], v)
+
+
+def test_optic_kernels_bit_exact_on_a_side_stream():
+    from types import SimpleNamespace
+    from flyhip.optic import OpticParams
+    stream = torch.cuda.Stream()
+    rng = np.random.default_rng(6)
+    with torch.cuda.stream(stream):
+        for B in (1, 3):
+            n = 259
+            v = (rng.random((B, n)) - 0.5).astype(np.float32)
+            adapt = rng.random((B, n)).astype(np.float32)
+            a = rng.random(n).astype(np.float32)
+            b = rng.random(n).astype(np.float32)
+            o = SimpleNamespace(v=T(v), adapt=T(adapt), B=B, n_rate=n, p=OpticParams(), _a=T(a), b_vec=T(b),
+                                _a_ad=0.997, _cuda_dr=torch.empty(B, n, device=DEV))
+            hip.optic_dr(o.v, o.b_vec, o._cuda_dr)
+            dr = oracle.optic_dr(v, b)
+            for _ in range(10):
+                y, pr, sk = (rng.random((B, n)).astype(np.float32) for _ in range(3))
+                hip.optic_update(o, T(y), T(pr), T(sk))
+                oracle.optic_update(v, adapt, dr, y, pr, sk, a, b, 1.0, 1.0, 0.997)
+            stream.synchronize()
+            assert_bits(N_(o.v), v, "v")
+            assert_bits(N_(o.adapt), adapt, "adapt")
+            assert_bits(N_(o._cuda_dr), dr, "dr")
+
+
+def test_photoreceptor_feedback_drive_bit_exact():
+    rng = np.random.default_rng(8)
+    B, n_pr = 3, 777
+    sens = rng.random((n_pr, 4)).astype(np.float32)
+    coefs = np.array([math.exp(-1), 1 - math.exp(-1), math.exp(-10 / 300), 1 - math.exp(-10 / 300), 0.02, 2.0], np.float32)
+    I_lp, I_mean = np.zeros((B, n_pr), np.float32), np.zeros((B, n_pr), np.float32)
+    fresh = np.ones(B, np.uint8)
+    tI_lp, tI_mean, tfresh = T(I_lp), T(I_mean), T(fresh)
+    a_pr = torch.empty(B, n_pr, device=DEV)
+    for _ in range(4):
+        rad = rng.random((B, n_pr, 4)).astype(np.float32)
+        hip.photoreceptor(T(rad), T(sens), tI_lp, tI_mean, tfresh, T(coefs), a_pr)
+        ref = oracle.photoreceptor(rad, sens, I_lp, I_mean, fresh, coefs)
+        assert_bits(N_(a_pr), ref, "contrast")
+        assert_bits(N_(tI_mean), I_mean, "I_mean")
+    assert int(tfresh.sum()) == 0
+    n = 900
+    rate = rng.uniform(0, 400, (B, n)).astype(np.float32)
+    spk = np.sort(rng.choice(n, 300, replace=False)).astype(np.int32)
+    s = torch.empty(B, 300, device=DEV)
+    hip.feedback(T(rate), T(spk), s)
+    assert_bits(N_(s), oracle.feedback(rate, spk), "feedback")
+    y = rng.normal(0, 0.5, (B, 300)).astype(np.float32)
+    drive = torch.zeros(B, n, device=DEV)
+    hip.drive_out(T(y), T(spk), drive, 100.0, 35.0)
+    assert_bits(N_(drive), oracle.drive_out(y, spk, n, 100.0, 35.0), "drive")
+
+
+def test_group_means():
+    rng = np.random.default_rng(9)
+    rates = rng.uniform(0, 100, (3, 50)).astype(np.float32)
+    ptr = np.array([0, 3, 3, 10], np.int32)
+    idx = rng.integers(0, 50, 10).astype(np.int32)
+    out = hip.group_means(T(ptr), T(idx), T(rates), T(np.ones((3, 50), np.float32)))
+    ref = oracle.group_means(ptr, idx, rates)
+    assert np.array_equal(N_(out)[:, :3], ref)
+    assert np.array_equal(N_(out)[:, 3], np.full(3, 50.0))
+
+
+def test_brain_raster_bit_exact_plain_lif():
+    """The plain-LIF workload's statistics at 3,000 neurons: Brain on the GPU against the oracle, 400 steps."""
+    c, inputs, hz = synth.plain_lif_graph(4, n=3000, n_inputs=60)
+    p = LIFParams.shiu(dt=0.1)
+    for B in (1, 2):
+        brain = Brain(c, p, device=DEV, batch=B, seed=123)
+        brain.record_activity = True
+        brain.set_poisson(inputs, hz)
+        from helpers import oracle_fly
+        om = oracle_fly(c, None, p, None, B, 123, None, with_optic=False)
+        om.st.poisson_p[:, inputs] = (hz * np.float32(p.dt / 1000.0)).astype(np.float32)
+        total = 0
+        for step, st in enumerate(om.lif_steps(400)):
+            brain.step()
+            s = N_(brain.spikes)
+            assert_bits(s, st.spikes, f"spikes B={B} step {step}")
+            total += s.sum()
+        for k in ("v", "g", "refrac", "rate", "spike_counts"):
+            assert_bits(N_(getattr(brain, k)), getattr(om.st, k), f"{k} B={B}")
+        assert total > 0
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_fly_frames_bit_exact_with_optic(graphs):
+    """A 2 % scale fly (LIF + optic lobe + photoreceptor movie): 5 frames of 10 ms, rasters identical step for step."""
+    conn, retina, layout, lp, op = fly_setup(0.02, seed=1)
+    B = 2
+    fb = FlyBrain(conn, retina, batch=B, device=DEV, seed=9, lif_params=lp, optic_params=op, cuda_graphs=graphs)
+    idx, hz = synth.fly_inputs(5, layout, 40)
+    fb.set_poisson(idx, hz)
+    fb.random_movie(17)
+    om = oracle_fly(conn, retina, lp, op, B, 9, 17)
+    om.st.poisson_p[:, idx] = (hz * np.float32(lp.dt / 1000.0)).astype(np.float32)
+    for frame in range(5):
+        raster = []
+        om.frame(100, raster)
+        fb.step(10.0)
+        assert_bits(N_(fb.brain.spikes), raster[-1], f"spikes frame {frame}")
+        assert_bits(N_(fb.brain.drive), om.st.drive, f"drive frame {frame}")
+        assert_bits(N_(fb.optic.v), om.os.v, f"optic v frame {frame}")
+        assert_bits(N_(fb.brain.spike_counts), om.st.spike_counts, f"counts frame {frame}")
+    for k in ("v", "g", "rate", "refrac"):
+        assert_bits(N_(getattr(fb.brain, k)), getattr(om.st, k), k)
+    assert om.st.spike_counts.sum() > 0
