@@ -1,0 +1,427 @@
+// Batched multi-fly brain step: 64 flies per wavefront, one lane per fly ("fly-minor" layout).
+//
+// State is (rows, 64): row r, lane b holds fly b's value of cell r, so a wavefront reads and writes one row of all
+// flies with one coalesced 256-byte access.  A spike is one bit: after a LIF step the wavefront's ballot of "fired"
+// is the 64-bit mask of which flies' copy of this cell spiked, stored in a ring of masks (ring_slots, rows).
+//
+// The synaptic delay (n_delay steps, 18 at dt 0.1 ms) makes every row independent for n_delay consecutive steps:
+// the input of step t comes from spikes of step t - n_delay, which a previous launch wrote.  So one launch runs up
+// to n_delay steps of every row with no synchronisation between wavefronts, and each row's membrane state stays in
+// registers across those steps.  The ring has ring_slots >= 2 * n_delay entries so the slots read during a launch
+// (steps T0 - D .. T0 + K - 1 - D) are never the slots written (T0 .. T0 + K - 1).
+//
+// Synaptic input keeps the reference CPU order (flyverse/brain.py:832-833, 679-692): g = g*a_s, then the in-edges
+// of the row in ascending presynaptic order, each adding w to the flies whose presynaptic bit is set.  The bit mask
+// is the select condition of the add (inverse ballot: an SGPR pair read as a per-lane predicate), so each edge costs
+// one add and one select for all 64 flies, and lanes whose bit is clear are untouched, exactly like the reference
+// skipping silent presynaptic cells.
+#include "common.h"
+
+namespace {
+
+// Constant-address-space views: a launch never writes the CSR or the ring slots it reads, so these reads may go
+// through the scalar cache (s_load with an SGPR offset: no per-edge address arithmetic).
+typedef const __attribute__((address_space(4))) uint32_t* cu32p;
+typedef const __attribute__((address_space(4))) float* cf32p;
+typedef const __attribute__((address_space(4))) uint64_t* cu64p;
+
+// acc + w[e] for every edge e in [e0, e1) whose presynaptic mask has this lane's bit, in edge order.  The mask is
+// the select condition (an SGPR pair), so a lane whose bit is clear keeps acc unchanged, as the reference skips a
+// silent presynaptic cell; 8 edges per round batch the scalar loads.
+__device__ __forceinline__ float fold_edges(float acc, cu32p off, cf32p w, const char* slot, int e0, int e1) {
+    int e = e0;
+    for (; e + 8 <= e1; e += 8) {
+        uint32_t o[8];
+        float wv[8];
+        uint64_t m[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) { o[j] = off[e + j]; wv[j] = w[e + j]; }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m[j] = *(cu64p)(slot + o[j]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            float t = acc + wv[j];
+            acc = __builtin_amdgcn_inverse_ballot_w64(m[j]) ? t : acc;
+        }
+    }
+    for (; e < e1; ++e) {
+        uint64_t m = *(cu64p)(slot + off[e]);
+        float t = acc + w[e];
+        acc = __builtin_amdgcn_inverse_ballot_w64(m) ? t : acc;
+    }
+    return acc;
+}
+
+struct BatchLif {
+    const int* ptr;          // (n_rows + 1) post-major CSR over LIF rows
+    const uint32_t* pre_off; // (nnz) byte offset of the presynaptic row's mask within a ring slot (row * 8)
+    const float* w;          // (nnz) weights (mV)
+    const int* sched;        // (n_rows) visiting order
+    const int* neuron_id;    // (n_rows) global neuron index (Philox counter)
+    const uint8_t* row_flags;// (n_rows) bit 0: some fly has Poisson forcing on this row
+    float *v, *g, *refrac, *rate, *counts;
+    const float *drive, *pois;
+    uint64_t* ring;
+    const uint64_t* seed;    // (64) per-lane Philox key
+    const uint32_t* fly_step;// (64) steps this lane's fly has taken at the start of the frame
+    const uint32_t* budget;  // (64) steps this lane's fly may take in total
+    const uint32_t* engine_step; // (1) engine step at the start of the frame (ring slot clock)
+    const float* P;          // (12) v_rest, v_reset, v_th, a_m, dt, t_ref, -, -, -, a_r, rate_gain, a_s
+    int n_rows, lanes, ring_slots, delay, record;
+};
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+__global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, int K) {
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * 4 + wave_id();
+    const int nw = gridDim.x * 4;
+    const uint32_t T0 = *a.engine_step + (uint32_t)step_offset;
+    const float v_rest = a.P[0], v_reset = a.P[1], v_th = a.P[2], am = a.P[3], dt = a.P[4], t_ref = a.P[5];
+    const float ar = a.P[9], rg = a.P[10], a_s = a.P[11];
+    const uint32_t R = (uint32_t)a.ring_slots, D = (uint32_t)a.delay;
+    uint64_t seed = 0;
+    uint32_t fstep = 0;
+    int live_steps = 0;
+    if (lane < a.lanes) {
+        seed = a.seed[lane];
+        fstep = a.fly_step[lane] + (uint32_t)step_offset;
+        uint32_t bud = a.budget[lane];
+        live_steps = bud > fstep ? (int)min(bud - fstep, (uint32_t)K) : 0;
+    }
+    for (int si = gw; si < a.n_rows; si += nw) {
+        const int r = a.sched[si];
+        const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
+        const uint32_t nid = (uint32_t)a.neuron_id[r];
+        const bool has_pois = a.row_flags[r] & 1;
+        const int64_t q = (int64_t)r * 64 + lane;
+        float v = a.v[q], g = a.g[q], ref = a.refrac[q], rate = a.rate[q];
+        float cnt = a.record ? a.counts[q] : 0.0f;
+        const float drive = a.drive[q];
+        const float p = has_pois ? a.pois[q] : 0.0f;
+        for (int k = 0; k < K; ++k) {
+            const uint32_t t = T0 + (uint32_t)k;
+            const char* rb = (const char*)(a.ring + (size_t)((t + R - D) % R) * a.n_rows);
+            float acc = g * a_s;
+            acc = fold_edges(acc, (cu32p)a.pre_off, (cf32p)a.w, rb, e0, e1);
+            // LIF update of flyverse/brain.py:838-868 with active = 1, no adaptation or depression
+            float target = v_rest + acc;
+            target = target + drive;
+            float dv = v - target;
+            dv = dv * am;
+            float vn = target + dv;
+            float rn = ref;
+            if (rn > 0.0f) vn = v_reset;
+            rn = rn - dt;
+            if (rn < 0.0f) rn = 0.0f;
+            float spike = (vn >= v_th) ? 1.0f : 0.0f;
+            if (p > 0.0f && fh_uniform(seed, fstep + (uint32_t)k, nid, 0u, 0u) < p) spike = 1.0f;
+            const bool fired = spike > 0.0f;
+            if (fired) { vn = v_reset; rn = t_ref; }
+            float rr = rate * ar;
+            float rs = spike * rg;
+            const bool live = k < live_steps;
+            if (live) {
+                g = acc; v = vn; ref = rn; rate = rr + rs;
+                cnt = cnt + spike;
+            }
+            const uint64_t bits = __builtin_amdgcn_ballot_w64(live && fired);
+            if (lane == 0) a.ring[(size_t)(t % R) * a.n_rows + r] = bits;
+        }
+        a.v[q] = v; a.g[q] = g; a.refrac[q] = ref; a.rate[q] = rate;
+        if (a.record) a.counts[q] = cnt;
+    }
+}
+
+// ------------------------------------------------------------------ optic lobe, fly-minor
+// Chunked CSR product over the (cols, 64) input: y[r, b] = scale * fold of chunk folds (csrc/spmv.hip order).
+__global__ __launch_bounds__(256) void k_spmm_fm(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                 const float* __restrict__ w, const int* __restrict__ sched,
+                                                 const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                 float scale) {
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * 4 + wave_id();
+    const int nw = gridDim.x * 4;
+    for (int si = gw; si < rows; si += nw) {
+        const int r = sched[si];
+        const int e0 = ptr[r], e1 = ptr[r + 1];
+        float total = 0.0f;
+        for (int cs = e0; cs < e1; cs += 16) {
+            const int ce = min(cs + 16, e1);
+            float s = 0.0f;
+            for (int e = cs; e < ce; ++e) {
+                float c = w[e] * x[(int64_t)idx[e] * 64 + lane];
+                s = s + c;
+            }
+            total = total + s;
+        }
+        if (scale != 1.0f) total = total * scale;
+        y[(int64_t)r * 64 + lane] = total;
+    }
+}
+
+// One optic substep fused with its recurrent product: y = W_rr dr_cur (chunked order), then the update of
+// flyverse/optic.py:525-551 (neural.cu:68-79) writing the next dr into dr_next; lanes of finished flies only copy.
+__global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                          const float* __restrict__ w, const int* __restrict__ sched,
+                                                          const float* __restrict__ dr_cur, float* __restrict__ dr_next,
+                                                          float* __restrict__ v, float* __restrict__ adapt,
+                                                          const float* __restrict__ pr, const float* __restrict__ spk,
+                                                          const float* __restrict__ a, const float* __restrict__ b,
+                                                          const uint64_t* __restrict__ live_mask, int rows, float gr,
+                                                          float ga, float aad) {
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * 4 + wave_id();
+    const int nw = gridDim.x * 4;
+    const bool live = (*live_mask >> lane) & 1ull;
+    for (int si = gw; si < rows; si += nw) {
+        const int r = sched[si];
+        const int e0 = ptr[r], e1 = ptr[r + 1];
+        float total = 0.0f;
+        for (int cs = e0; cs < e1; cs += 16) {
+            const int ce = min(cs + 16, e1);
+            float s = 0.0f;
+            for (int e = cs; e < ce; ++e) {
+                float c = w[e] * dr_cur[(int64_t)idx[e] * 64 + lane];
+                s = s + c;
+            }
+            total = total + s;
+        }
+        const int64_t q = (int64_t)r * 64 + lane;
+        const float old = dr_cur[q];
+        if (!live) { dr_next[q] = old; continue; }
+        float inp = gr * total;
+        inp = inp + pr[q];
+        float gad = ga * adapt[q];
+        inp = inp - gad;
+        inp = inp + spk[q];
+        float d = v[q] - inp;
+        d = d * a[r];
+        float vn = inp + d;
+        v[q] = vn;
+        float da = adapt[q] - old;
+        da = da * aad;
+        adapt[q] = old + da;
+        float s2 = vn + b[r];
+        dr_next[q] = fh_clip01(s2) - b[r];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_optic_dr_fm(const float* __restrict__ v, const float* __restrict__ b,
+                                                     float* __restrict__ dr, int64_t total) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    float base = b[t >> 6];
+    float s = v[t] + base;
+    dr[t] = fh_clip01(s) - base;
+}
+
+// Photoreceptor stage with the radiance drawn in-kernel: rad[p, c] of lane b = Philox(movie_seed[b], frame[b],
+// p*4 + c, 0, stream 1), the same uniform the plain path's launch_uniform gives fly 0 of a one-fly batch.
+__global__ __launch_bounds__(256) void k_photoreceptor_fm(const float* __restrict__ sens, float* __restrict__ I_lp,
+                                                          float* __restrict__ I_mean, float* __restrict__ a_pr,
+                                                          const uint64_t* __restrict__ movie_seed,
+                                                          const uint32_t* __restrict__ frame,
+                                                          const uint64_t* __restrict__ fresh_mask,
+                                                          const uint64_t* __restrict__ live_mask,
+                                                          const float* __restrict__ cf, int n_pr) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_pr * 64) return;
+    const int lane = (int)(t & 63), p = (int)(t >> 6);
+    if (!((*live_mask >> lane) & 1ull)) return;
+    const uint64_t ms = movie_seed[lane];
+    const uint32_t fr = frame[lane];
+    float I = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float rad = fh_uniform(ms, fr, (uint32_t)(p * 4 + c), 0u, 1u);
+        float q = rad * sens[p * 4 + c];
+        I = I + q;
+    }
+    const bool fr0 = (*fresh_mask >> lane) & 1ull;
+    float lp = fr0 ? I : I_lp[t];
+    float mean = fr0 ? I : I_mean[t];
+    float l1 = cf[0] * lp, l2 = cf[1] * I;
+    lp = l1 + l2;
+    float m1 = cf[2] * mean, m2 = cf[3] * lp;
+    mean = m1 + m2;
+    I_lp[t] = lp; I_mean[t] = mean;
+    float num = lp - mean, den = mean + cf[4];
+    float c = num / den;
+    c = c < -1.0f ? -1.0f : c;
+    c = c > cf[5] ? cf[5] : c;
+    a_pr[t] = c;
+}
+
+__global__ __launch_bounds__(256) void k_feedback_fm(const float* __restrict__ rate, const int* __restrict__ spk_row,
+                                                     float* __restrict__ s, int64_t total) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    float q = rate[(int64_t)spk_row[t >> 6] * 64 + (t & 63)] / 100.0f;
+    q = q < 0.0f ? 0.0f : q;
+    q = q > 3.0f ? 3.0f : q;
+    s[t] = q;
+}
+
+__global__ __launch_bounds__(256) void k_drive_out_fm(const float* __restrict__ y, const int* __restrict__ spk_row,
+                                                      float* __restrict__ drive, float gain_out, float clip,
+                                                      int64_t total) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    float d = gain_out * y[t];
+    d = d < -clip ? -clip : d;
+    d = d > clip ? clip : d;
+    drive[(int64_t)spk_row[t >> 6] * 64 + (t & 63)] = d;
+}
+
+// End of a frame: the engine clock and every live fly advance by `steps`; live and fresh masks are refreshed.
+__global__ void k_frame_end(uint32_t* engine_step, uint32_t* fly_step, const uint32_t* budget, uint32_t* frame,
+                            uint64_t* fresh_mask, uint64_t* live_mask, int lanes, int steps) {
+    const int lane = threadIdx.x;
+    bool live_next = false;
+    if (lane < lanes) {
+        uint32_t fs = fly_step[lane], bud = budget[lane];
+        if (fs < bud) {
+            fs = fs + min((uint32_t)steps, bud - fs);
+            fly_step[lane] = fs;
+            frame[lane] = frame[lane] + 1;
+        }
+        live_next = fs < bud;
+    }
+    const uint64_t live = __builtin_amdgcn_ballot_w64(live_next);
+    if (lane == 0) {
+        *engine_step = *engine_step + (uint32_t)steps;
+        *live_mask = live;
+        *fresh_mask = 0;
+    }
+}
+
+// Start new flies on the lanes of `mask`: LIF and optic state to rest, their spike bits cleared from every ring
+// slot, photoreceptor stage marked fresh.  Per-lane seeds, budgets and step counters are written by the caller.
+__global__ __launch_bounds__(256) void k_reset_lanes(float* v, float* g, float* refrac, float* rate, float* counts,
+                                                     float* drive, float* pois, float* ov, float* oadapt, uint64_t* ring,
+                                                     uint64_t* fresh_mask, uint64_t* live_mask, uint64_t mask,
+                                                     uint64_t live_set, float v_rest, int n_rows, int n_rate,
+                                                     int ring_slots) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)(t & 63);
+    const bool mine = (mask >> lane) & 1ull;
+    const int64_t lif = (int64_t)n_rows * 64, opt = (int64_t)n_rate * 64;
+    if (t < lif && mine) {
+        v[t] = v_rest; g[t] = 0.0f; refrac[t] = 0.0f; rate[t] = 0.0f; counts[t] = 0.0f; drive[t] = 0.0f;
+        pois[t] = 0.0f;
+    }
+    if (t < opt && mine) { ov[t] = 0.0f; oadapt[t] = 0.0f; }
+    const int64_t words = (int64_t)n_rows * ring_slots;
+    for (int64_t k = t; k < words; k += (int64_t)gridDim.x * blockDim.x) ring[k] &= ~mask;
+    if (t == 0) {
+        *fresh_mask |= mask;
+        *live_mask = (*live_mask & ~mask) | (live_set & mask);
+    }
+}
+
+inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
+
+}  // namespace
+
+// Up to `delay` LIF steps of every row for 64 flies.  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v, g,
+// refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12)];
+// dims = [n_rows, lanes, ring_slots, delay, record, grid].
+FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream) {
+    BatchLif a;
+    a.ptr = (const int*)p[0]; a.pre_off = (const uint32_t*)p[1]; a.w = (const float*)p[2]; a.sched = (const int*)p[3];
+    a.neuron_id = (const int*)p[4]; a.row_flags = (const uint8_t*)p[5];
+    a.v = (float*)p[6]; a.g = (float*)p[7]; a.refrac = (float*)p[8]; a.rate = (float*)p[9]; a.counts = (float*)p[10];
+    a.drive = (const float*)p[11]; a.pois = (const float*)p[12]; a.ring = (uint64_t*)p[13];
+    a.seed = (const uint64_t*)p[14]; a.fly_step = (const uint32_t*)p[15]; a.budget = (const uint32_t*)p[16];
+    a.engine_step = (const uint32_t*)p[17]; a.P = (const float*)p[18];
+    a.n_rows = dims[0]; a.lanes = dims[1]; a.ring_slots = dims[2]; a.delay = dims[3]; a.record = dims[4];
+    const int grid = dims[5];
+    if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
+        grid <= 0)
+        return FH_ERR_ARGS;
+    if (a.n_rows == 0 || K == 0) return 0;
+    hipLaunchKernelGGL(k_batch_lif, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
+    FH_LAUNCH_CHECK();
+}
+
+// y (rows, 64) = scale * W x (cols, 64).  p = [ptr, idx, w, sched, x, y].
+FH_EXPORT int launch_spmm_fm(void** p, int rows, float scale, int grid, void* stream) {
+    if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
+    if (!rows) return 0;
+    hipLaunchKernelGGL(k_spmm_fm, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const int*)p[0], (const int*)p[1],
+                       (const float*)p[2], (const int*)p[3], (const float*)p[4], (float*)p[5], rows, scale);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [ptr, idx, w, sched, dr_cur, dr_next, v, adapt, pr, spk, a, b, live_mask].
+FH_EXPORT int launch_optic_substep_fm(void** p, int rows, float gr, float ga, float aad, int grid, void* stream) {
+    if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
+    if (!rows) return 0;
+    hipLaunchKernelGGL(k_optic_substep_fm, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
+                       (const int*)p[1], (const float*)p[2], (const int*)p[3], (const float*)p[4], (float*)p[5],
+                       (float*)p[6], (float*)p[7], (const float*)p[8], (const float*)p[9], (const float*)p[10],
+                       (const float*)p[11], (const uint64_t*)p[12], rows, gr, ga, aad);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [v, b, dr], (rows, 64).
+FH_EXPORT int launch_optic_dr_fm(void** p, int rows, void* stream) {
+    if (rows < 0) return FH_ERR_ARGS;
+    if (!rows) return 0;
+    int64_t total = (int64_t)rows * 64;
+    hipLaunchKernelGGL(k_optic_dr_fm, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
+                       (const float*)p[1], (float*)p[2], total);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [sens, I_lp, I_mean, a_pr, movie_seed, frame, fresh_mask, live_mask, coefs].
+FH_EXPORT int launch_photoreceptor_fm(void** p, int n_pr, void* stream) {
+    if (n_pr < 0) return FH_ERR_ARGS;
+    if (!n_pr) return 0;
+    hipLaunchKernelGGL(k_photoreceptor_fm, dim3(nblk((int64_t)n_pr * 64)), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)p[0], (float*)p[1], (float*)p[2], (float*)p[3], (const uint64_t*)p[4],
+                       (const uint32_t*)p[5], (const uint64_t*)p[6], (const uint64_t*)p[7], (const float*)p[8], n_pr);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [rate (rows, 64), spk_row (n_spk), s (n_spk, 64)].
+FH_EXPORT int launch_feedback_fm(void** p, int n_spk, void* stream) {
+    if (n_spk < 0) return FH_ERR_ARGS;
+    if (!n_spk) return 0;
+    int64_t total = (int64_t)n_spk * 64;
+    hipLaunchKernelGGL(k_feedback_fm, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
+                       (const int*)p[1], (float*)p[2], total);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [y_sr (n_spk, 64), spk_row, drive (rows, 64)].
+FH_EXPORT int launch_drive_out_fm(void** p, int n_spk, float gain_out, float clip, void* stream) {
+    if (n_spk < 0) return FH_ERR_ARGS;
+    if (!n_spk) return 0;
+    int64_t total = (int64_t)n_spk * 64;
+    hipLaunchKernelGGL(k_drive_out_fm, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
+                       (const int*)p[1], (float*)p[2], gain_out, clip, total);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [engine_step, fly_step, budget, frame, fresh_mask, live_mask].
+FH_EXPORT int launch_frame_end(void** p, int lanes, int steps, void* stream) {
+    if (lanes < 0 || lanes > 64 || steps < 0) return FH_ERR_ARGS;
+    hipLaunchKernelGGL(k_frame_end, dim3(1), dim3(64), 0, (hipStream_t)stream, (uint32_t*)p[0], (uint32_t*)p[1],
+                       (const uint32_t*)p[2], (uint32_t*)p[3], (uint64_t*)p[4], (uint64_t*)p[5], lanes, steps);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [v, g, refrac, rate, counts, drive, pois, optic v, optic adapt, ring, fresh_mask, live_mask].
+FH_EXPORT int launch_reset_lanes(void** p, uint64_t mask, uint64_t live_set, float v_rest, int n_rows, int n_rate,
+                                 int ring_slots, void* stream) {
+    if (n_rows < 0 || n_rate < 0 || ring_slots < 0) return FH_ERR_ARGS;
+    int64_t total = (int64_t)(n_rows > n_rate ? n_rows : n_rate) * 64;
+    if (total < 64) total = 64;
+    hipLaunchKernelGGL(k_reset_lanes, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, (float*)p[0],
+                       (float*)p[1], (float*)p[2], (float*)p[3], (float*)p[4], (float*)p[5], (float*)p[6],
+                       (float*)p[7], (float*)p[8], (uint64_t*)p[9], (uint64_t*)p[10], (uint64_t*)p[11], mask, live_set,
+                       v_rest, n_rows, n_rate, ring_slots);
+    FH_LAUNCH_CHECK();
+}
