@@ -1,0 +1,243 @@
+"""Brain-step benchmark: batched sim-s/wall-s at 167k neurons (B = 64) against the plain single-fly path (B = 1).
+
+Workload (synthetic, seeded): the full-scale fly graph (167,106 neurons, ~25.4 M edges, flyhip.synth.fly_graph),
+Shiu LIF at dt 0.1 ms coupled to the 89,390-unit optic lobe every 10 ms frame (10 optic substeps of 1 ms).
+  * B = 64: flyhip.batch.BatchBrain, 64 lanes fed from a queue of rollouts, each with its own Poisson seed,
+    photoreceptor movie seed, 300 Poisson inputs at U[0, 200] Hz and a length uniform over 2,000-10,000 LIF steps
+    (whole 10 ms frames); finished lanes are refilled.  One bench step = one 10 ms frame of all 64 lanes, replayed
+    from a captured HIP graph.  value = simulated fly-seconds / wall-seconds over the timed frames.
+  * B = 1: flyhip.FlyBrain(batch=1, cuda_graphs=True), the plain kernels (launch_pull + launch_lif_rng per step),
+    one captured graph per frame shape; reported as b1_sim_s_per_wall_s and the ratio.
+  * roofline: the batched LIF launch (csrc/batch.hip k_batch_lif, <= 18 steps per launch), timed with HIP events
+    on its stream after the timed region; algorithmic bytes per launch from the per-step LIF and synaptic-input
+    byte counts.
+  * cpu_baseline: the CPU oracle (oracle/, plain C, one thread) on one fly of the same workload for a bounded sample
+    of frames, scaled to sim-s/wall-s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "flyverse-core_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_rollouts(layout, n, seed, frame_steps=100, lo=2000, hi=10000):
+    from flyhip import synth
+    from flyhip.batch import Rollout
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        idx, hz = synth.fly_inputs(int(rng.integers(0, 2**31)), layout, 300, 200.0)
+        frames = int(rng.integers(lo // frame_steps, hi // frame_steps + 1))
+        out.append(Rollout(seed=int(rng.integers(1, 2**62)), movie_seed=int(rng.integers(1, 2**62)),
+                           steps=frames * frame_steps, input_idx=idx, input_hz=hz))
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def lif_block_bytes(bb, K, B):
+    """Algorithmic bytes of one k_batch_lif launch of K steps for B live flies: per step, the LIF state traffic
+    B*N_act*(32 + 8 + 1/8) and the synaptic input 8 * sum of out-degrees over the presynaptic cells that fired in
+    any fly + B*N_act/8 for the spike bits (the per-step counts of the measurement plan).  The firing set is
+    measured over the steps held in the ring."""
+    import torch
+    T = int(bb.engine_step.item())
+    n = min(bb.ring_slots, T)
+    ring = bb.ring.cpu().numpy().view(np.uint64)
+    outdeg = np.bincount(bb.W_lif.indices, minlength=bb.n_rows).astype(np.float64)
+    touched = []
+    for t in range(T - n, T):
+        fired = ring[t % bb.ring_slots] != 0
+        touched.append(outdeg[fired].sum())
+    N = bb.n_rows
+    per_step = B * N * (32 + 8 + 1 / 8) + 8 * float(np.mean(touched)) + B * N / 8
+    act = float(np.mean([np.unpackbits(ring[t % bb.ring_slots].view(np.uint8)).sum() for t in range(T - n, T)]))
+    return K * per_step, act / (B * N)
+
+
+def time_lif_launches(bb, n=6):
+    """Average duration (ms) of k_batch_lif launches of n_delay steps, with HIP events on the launch stream."""
+    import ctypes as ct
+    import torch
+    from flyhip.batch import _lib
+    lif = [bb.lif_ptr, bb.lif_pre_off, bb.lif_w, bb.lif_sched, bb.neuron_id, bb.row_flags, bb.v, bb.g, bb.refrac,
+           bb.rate, bb.counts, bb.drive, bb.pois, bb.ring, bb.seed, bb.fly_step, bb.budget, bb.engine_step, bb.P]
+    arr = (ct.c_void_p * len(lif))(*(t.data_ptr() for t in lif))
+    stream = torch.cuda.current_stream()
+    fn = _lib(bb.device).launch_batch_lif
+    times = []
+    # launches at offsets 0, n_delay, ... of the next frame; the state just advances (the bench is over)
+    for i in range(n):
+        off = (i % max(1, bb.frame_steps // bb.n_delay)) * bb.n_delay
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        err = fn(arr, bb._dims, off, bb.n_delay, stream.cuda_stream)
+        e1.record(stream)
+        if err:
+            raise RuntimeError(f"launch_batch_lif: {err}")
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    return float(np.mean(times[1:])) if len(times) > 1 else times[0]
+
+
+def cpu_baseline(conn, retina, lp, op, rollout, frames):
+    """The CPU oracle (plain C, one thread) running one fly of the workload for `frames` frames."""
+    sys.path.insert(0, str(ROOT))
+    sys.path.insert(0, str(ROOT / "tests"))
+    from helpers import oracle_fly
+    om = oracle_fly(conn, retina, lp, op, 1, [rollout.seed], [rollout.movie_seed])
+    om.st.poisson_p[0, rollout.input_idx] = (rollout.input_hz * np.float32(lp.dt / 1000.0)).astype(np.float32)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        om.frame(100)
+    wall = time.perf_counter() - t0
+    return frames * 0.010 / wall, wall
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60, help="timed 10 ms frames of the B=64 engine")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--b1-frames", type=int, default=30)
+    ap.add_argument("--cpu-frames", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device copy (the reference kernel for PMC byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this workload shards by fly with no communication, and runs on more than "
+              f"one GPU are not part of this build (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from flyhip import synth
+    from flyhip.batch import BatchBrain
+    from flyhip.brain import LIFParams
+    from flyhip.fly import FlyBrain
+    from flyhip.optic import OpticParams
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("building the full-scale graph ...")
+    conn, retina, layout = synth.fly_graph(1, 1.0)
+    lp = LIFParams.shiu(dt=0.1)
+    op = OpticParams(tau_by_type=conn.rate_tau, baseline_by_type=conn.rate_baseline, pair_gain=[])
+    rollouts = make_rollouts(layout, 64 + 4 * 64, seed=11)
+
+    # ---------------------------------------------------------------- B = 64, batched engine
+    log("B=64 batched engine: setup")
+    bb = BatchBrain(conn, retina, lp, op, device=dev)
+    bb.submit(rollouts)
+    bb.prepare(graph=True)
+    bb.run(args.warmup)
+    torch.cuda.synchronize()
+    s0 = bb.simulated_steps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    bb.run(args.steps)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    sim_steps = bb.simulated_steps - s0
+    value = sim_steps * lp.dt / 1000.0 / (gpu_ms / 1000.0)
+    log(f"B=64: {args.steps} frames, {sim_steps} fly-steps in {gpu_ms:.1f} ms (host wall {wall*1e3:.1f} ms): "
+        f"{value:.3f} sim-s/wall-s; rollouts completed {bb.completed}")
+
+    K = bb.n_delay
+    lif_ms = time_lif_launches(bb)
+    alg_bytes, activity = lif_block_bytes(bb, K, 64)
+    achieved = alg_bytes / (lif_ms / 1e3) / 1e9
+    log(f"k_batch_lif: {lif_ms:.3f} ms per {K}-step launch, activity {activity*100:.3f} % per step, "
+        f"{alg_bytes/1e6:.1f} MB algorithmic -> {achieved:.0f} GB/s")
+    del bb
+    torch.cuda.empty_cache()
+
+    # ---------------------------------------------------------------- B = 1, plain kernels
+    log("B=1 plain path: setup")
+    r = rollouts[0]
+    fb = FlyBrain(conn, retina, batch=1, device=dev, seed=r.seed, lif_params=lp, optic_params=op, cuda_graphs=True)
+    fb.set_poisson(r.input_idx, r.input_hz)
+    fb.random_movie(r.movie_seed)
+    for _ in range(10):                      # every delay-ring phase of a 100-step frame captured once
+        fb.step(10.0)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.b1_frames):
+        fb.step(10.0)
+    e1.record()
+    torch.cuda.synchronize()
+    b1_ms = e0.elapsed_time(e1)
+    b1 = args.b1_frames * 0.010 / (b1_ms / 1000.0)
+    log(f"B=1: {args.b1_frames} frames in {b1_ms:.1f} ms: {b1:.3f} sim-s/wall-s")
+    del fb
+    torch.cuda.empty_cache()
+
+    if args.copy_calibration:
+        x = torch.empty(2**28, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+
+    # ---------------------------------------------------------------- CPU oracle on the host cores
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU oracle: {args.cpu_frames} frames of one fly ...")
+        rate, cwall = cpu_baseline(conn, retina, lp, op, rollouts[0], args.cpu_frames)
+        cpu = {"value": round(rate, 6), "unit": "sim-s/wall-s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_frames} frames (= {args.cpu_frames * 100} LIF steps, {args.cpu_frames * 10} optic "
+                         f"substeps) of one fly of the same workload on the C oracle, {cwall:.1f} s wall; "
+                         f"host: {cpu_model()}, os.cpu_count()={os.cpu_count()}, "
+                         f"affinity={len(os.sched_getaffinity(0))}; B=64 on CPU is 64x this (linear)"}
+        log(f"CPU: {rate:.5f} sim-s/wall-s ({cwall:.1f} s)")
+
+    line = {
+        "metric": "batched sim-s/wall-s at 167k neurons (B=64) vs B=1; OL fwd+bwd ms/step",
+        "value": round(value, 4), "unit": "sim-s/wall-s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(gpu_ms / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "full-scale fly (167,106 neurons, 25.4M edges, LIF dt 0.1 ms + 89,390-unit optic lobe), "
+                               "B=64 batched, rollouts uniform 2,000-10,000 steps with lane refill; one step = one "
+                               "10 ms frame",
+                   "batch": 64, "neurons": int(conn.n), "edges": int(conn.W.nnz), "dt_ms": lp.dt,
+                   "b1_sim_s_per_wall_s": round(b1, 4), "ratio_b64_vs_b1": round(value / b1, 2),
+                   "b1_ms_per_frame": round(b1_ms / args.b1_frames, 4), "spike_activity_per_step": activity},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_batch_lif", "launch_ms": round(lif_ms, 4), "steps_per_launch": K,
+                     "algorithmic_bytes_per_launch": int(alg_bytes)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

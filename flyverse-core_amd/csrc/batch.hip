@@ -52,6 +52,94 @@ __device__ __forceinline__ float fold_edges(float acc, cu32p off, cf32p w, const
     return acc;
 }
 
+// The same fold with the masks gathered by vector loads: lane j loads the mask of edge j of a 64-edge chunk (one
+// instruction for 64 scattered 8-byte reads through the vector cache), then each edge's mask is read back into an
+// SGPR pair with v_readlane and used as the select condition, in edge order.
+__device__ __forceinline__ float fold_edges_gather(float acc, const uint32_t* __restrict__ off, cf32p w,
+                                                   const char* slot, int e0, int e1, int lane) {
+    for (int cb = e0; cb < e1; cb += 64) {
+        const int e = cb + lane;
+        uint64_t mj = 0;
+        if (e < e1) mj = *(const uint64_t*)(slot + off[e]);
+        const uint32_t lo = (uint32_t)mj, hi = (uint32_t)(mj >> 32);
+        const int n = min(64, e1 - cb);
+        int j = 0;
+        for (; j + 8 <= n; j += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint64_t m = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)lo, j + u) |
+                                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, j + u) << 32);
+                float t = acc + w[cb + j + u];
+                acc = __builtin_amdgcn_inverse_ballot_w64(m) ? t : acc;
+            }
+        }
+        for (; j < n; ++j) {
+            const uint64_t m = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)lo, j) |
+                               ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, j) << 32);
+            float t = acc + w[cb + j];
+            acc = __builtin_amdgcn_inverse_ballot_w64(m) ? t : acc;
+        }
+    }
+    return acc;
+}
+
+// Transpose of the 32x32 bit block held by each half-wave (lane = row, bit = column), in place: five exchange stages
+// with the lane at distance s (ds_swizzle, xor mode), each moving the off-diagonal s x s bit blocks with one rotate
+// (v_alignbit) and one bit-field insert (v_bfi).
+template <int S, uint32_t M>
+__device__ __forceinline__ uint32_t transpose_stage(uint32_t x, int lane) {
+    const uint32_t t = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (S << 10) | 0x1F);
+    const bool lower = (lane & S) != 0;
+    const uint32_t ts = __builtin_amdgcn_alignbit(t, t, lower ? S : 32 - S);     // rotr(t, S) or rotl(t, S)
+    const uint32_t keep = lower ? ~M : M;
+    return (x & keep) | (ts & ~keep);
+}
+
+__device__ __forceinline__ uint32_t transpose32(uint32_t x, int lane) {
+    x = transpose_stage<16, 0x0000FFFFu>(x, lane);
+    x = transpose_stage<8, 0x00FF00FFu>(x, lane);
+    x = transpose_stage<4, 0x0F0F0F0Fu>(x, lane);
+    x = transpose_stage<2, 0x33333333u>(x, lane);
+    x = transpose_stage<1, 0x55555555u>(x, lane);
+    return x;
+}
+
+// The fold with the 64 x 64 bit matrix of a 64-edge chunk transposed: lane j gathers the mask of edge j (bit b =
+// fly b's copy of the presynaptic cell fired); after the transpose lane b holds the bits of its own fly over the 64
+// edges, and walks its set bits in ascending edge order adding the staged weights.  The work per chunk is the
+// transpose plus as many rounds as the busiest lane has set bits, instead of two lane reads per edge.
+__device__ __forceinline__ float fold_edges_transposed(float acc, const uint32_t* __restrict__ off,
+                                                       const float* __restrict__ w, const char* slot, int e0, int e1,
+                                                       int lane, float* wlds) {
+    for (int cb = e0; cb < e1; cb += 64) {
+        const int e = cb + lane;
+        uint64_t mj = 0;
+        float wj = 0.0f;
+        if (e < e1) {
+            mj = *(const uint64_t*)(slot + off[e]);
+            wj = w[e];
+        }
+        wlds[lane] = wj;
+        uint32_t lo = (uint32_t)mj, hi = (uint32_t)(mj >> 32);
+        // 64 x 64 = four 32 x 32 blocks: swap the off-diagonal halves between lane j and j ^ 32, then transpose each
+        const uint32_t send = lane < 32 ? hi : lo;
+        const uint32_t got = (uint32_t)__shfl_xor((int)send, 32);
+        if (lane < 32) hi = got; else lo = got;
+        lo = transpose32(lo, lane);
+        hi = transpose32(hi, lane);
+        __builtin_amdgcn_wave_barrier();
+        while (__builtin_amdgcn_ballot_w64((lo | hi) != 0u)) {
+            if ((lo | hi) != 0u) {
+                const int j = lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi);
+                acc = acc + wlds[j];
+                if (lo) lo &= lo - 1u; else hi &= hi - 1u;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    return acc;
+}
+
 struct BatchLif {
     const int* ptr;          // (n_rows + 1) post-major CSR over LIF rows
     const uint32_t* pre_off; // (nnz) byte offset of the presynaptic row's mask within a ring slot (row * 8)
@@ -67,12 +155,13 @@ struct BatchLif {
     const uint32_t* budget;  // (64) steps this lane's fly may take in total
     const uint32_t* engine_step; // (1) engine step at the start of the frame (ring slot clock)
     const float* P;          // (12) v_rest, v_reset, v_th, a_m, dt, t_ref, -, -, -, a_r, rate_gain, a_s
-    int n_rows, lanes, ring_slots, delay, record;
+    int n_rows, lanes, ring_slots, delay, record, mode;
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, int K) {
+    __shared__ float wstage[4][64];
     const int lane = threadIdx.x & 63;
     const int gw = blockIdx.x * 4 + wave_id();
     const int nw = gridDim.x * 4;
@@ -103,7 +192,12 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
             const uint32_t t = T0 + (uint32_t)k;
             const char* rb = (const char*)(a.ring + (size_t)((t + R - D) % R) * a.n_rows);
             float acc = g * a_s;
-            acc = fold_edges(acc, (cu32p)a.pre_off, (cf32p)a.w, rb, e0, e1);
+            if (a.mode == 2)
+                acc = fold_edges_transposed(acc, a.pre_off, a.w, rb, e0, e1, lane, wstage[threadIdx.x >> 6]);
+            else if (a.mode == 1)
+                acc = fold_edges_gather(acc, a.pre_off, (cf32p)a.w, rb, e0, e1, lane);
+            else
+                acc = fold_edges(acc, (cu32p)a.pre_off, (cf32p)a.w, rb, e0, e1);
             // LIF update of flyverse/brain.py:838-868 with active = 1, no adaptation or depression
             float target = v_rest + acc;
             target = target + drive;
@@ -134,6 +228,41 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
 }
 
 // ------------------------------------------------------------------ optic lobe, fly-minor
+// One row of a chunked product over a (cols, 64) input: all 16 loads of a chunk are issued before its fold, so a
+// wavefront waits once per chunk instead of once per entry.
+__device__ __forceinline__ float fold_row_fm(cu32p idx, cf32p w, const float* __restrict__ x, int e0, int e1,
+                                             int lane) {
+    float total = 0.0f;
+    int cs = e0;
+    for (; cs + 16 <= e1; cs += 16) {
+        uint32_t c[16];
+        float wv[16], xv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) { c[j] = idx[cs + j]; wv[j] = w[cs + j]; }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) xv[j] = x[(size_t)c[j] * 64 + lane];
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) { float p = wv[j] * xv[j]; s = s + p; }
+        total = total + s;
+    }
+    if (cs < e1) {
+        const int n = e1 - cs;
+        float wv[16], xv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < n) { wv[j] = w[cs + j]; xv[j] = x[(size_t)idx[cs + j] * 64 + lane]; }
+        }
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < n) { float p = wv[j] * xv[j]; s = s + p; }
+        }
+        total = total + s;
+    }
+    return total;
+}
+
 // Chunked CSR product over the (cols, 64) input: y[r, b] = scale * fold of chunk folds (csrc/spmv.hip order).
 __global__ __launch_bounds__(256) void k_spmm_fm(const int* __restrict__ ptr, const int* __restrict__ idx,
                                                  const float* __restrict__ w, const int* __restrict__ sched,
@@ -144,17 +273,7 @@ __global__ __launch_bounds__(256) void k_spmm_fm(const int* __restrict__ ptr, co
     const int nw = gridDim.x * 4;
     for (int si = gw; si < rows; si += nw) {
         const int r = sched[si];
-        const int e0 = ptr[r], e1 = ptr[r + 1];
-        float total = 0.0f;
-        for (int cs = e0; cs < e1; cs += 16) {
-            const int ce = min(cs + 16, e1);
-            float s = 0.0f;
-            for (int e = cs; e < ce; ++e) {
-                float c = w[e] * x[(int64_t)idx[e] * 64 + lane];
-                s = s + c;
-            }
-            total = total + s;
-        }
+        float total = fold_row_fm((cu32p)idx, (cf32p)w, x, ptr[r], ptr[r + 1], lane);
         if (scale != 1.0f) total = total * scale;
         y[(int64_t)r * 64 + lane] = total;
     }
@@ -176,17 +295,7 @@ __global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict_
     const bool live = (*live_mask >> lane) & 1ull;
     for (int si = gw; si < rows; si += nw) {
         const int r = sched[si];
-        const int e0 = ptr[r], e1 = ptr[r + 1];
-        float total = 0.0f;
-        for (int cs = e0; cs < e1; cs += 16) {
-            const int ce = min(cs + 16, e1);
-            float s = 0.0f;
-            for (int e = cs; e < ce; ++e) {
-                float c = w[e] * dr_cur[(int64_t)idx[e] * 64 + lane];
-                s = s + c;
-            }
-            total = total + s;
-        }
+        const float total = fold_row_fm((cu32p)idx, (cf32p)w, dr_cur, ptr[r], ptr[r + 1], lane);
         const int64_t q = (int64_t)r * 64 + lane;
         const float old = dr_cur[q];
         if (!live) { dr_next[q] = old; continue; }
@@ -326,7 +435,9 @@ inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
 
 // Up to `delay` LIF steps of every row for 64 flies.  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v, g,
 // refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12)];
-// dims = [n_rows, lanes, ring_slots, delay, record, grid].
+// dims = [n_rows, lanes, ring_slots, delay, record, grid, mode]; mode 0 reads the spike masks through the scalar
+// cache, mode 1 gathers them with vector loads, mode 2 gathers and transposes them (fold_edges,
+// fold_edges_gather, fold_edges_transposed: the same additions in the same order, so the same result).
 FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream) {
     BatchLif a;
     a.ptr = (const int*)p[0]; a.pre_off = (const uint32_t*)p[1]; a.w = (const float*)p[2]; a.sched = (const int*)p[3];
@@ -337,6 +448,7 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     a.engine_step = (const uint32_t*)p[17]; a.P = (const float*)p[18];
     a.n_rows = dims[0]; a.lanes = dims[1]; a.ring_slots = dims[2]; a.delay = dims[3]; a.record = dims[4];
     const int grid = dims[5];
+    a.mode = dims[6];
     if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
         grid <= 0)
         return FH_ERR_ARGS;

@@ -29,11 +29,11 @@ def rollouts(layout, n, frames_lo, frames_hi, frame_steps, seed=0):
     return out
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_lanes_equal_solo_oracle_runs_with_refill(graph):
+@pytest.mark.parametrize("graph,mode", [(False, 0), (True, 1), (True, 2), (False, 2)])
+def test_lanes_equal_solo_oracle_runs_with_refill(graph, mode):
     conn, retina, layout, lp, op = fly_setup(0.02, seed=2)
     frame_ms, frame_steps = 1.0, 10           # 10 LIF steps and one optic substep per frame: the ring holds a frame
-    bb = BatchBrain(conn, retina, lp, op, device=DEV, frame_ms=frame_ms, grid=64)
+    bb = BatchBrain(conn, retina, lp, op, device=DEV, frame_ms=frame_ms, grid=64, fold_mode=mode)
     rs = rollouts(layout, 70, 3, 5, frame_steps, seed=1)    # 70 rollouts > 64 lanes: six lanes are refilled
     rs[0].steps = 2 * frame_steps                             # lane 0 finishes first and takes rollout 64
     bb.submit(rs)
