@@ -1,0 +1,373 @@
+// Optic-lobe rate network: forward substeps and the hand-written adjoint (an addition: the reference has no
+// adjoint, only torch autograd through OpticLobe._step_frame_grad, flyverse/optic.py:606-676).
+//
+// Model per step t (flyverse/optic.py:525-551 with no spiking feedback), units i, movies b:
+//   dr_t  = clip(v_t + b_i, 0, 1) - b_i
+//   y_t   = W dr_t,            W_ij = G[type_i, type_j] * W0_ij
+//   inp_t = g_rr*y_t + P_t - g_ad*adapt_t
+//   v_t+1 = inp_t + (v_t - inp_t)*a_i,   a_i = exp(-dt/tau_type_i)
+//   adapt_t+1 = dr_t + (adapt_t - dr_t)*a_ad
+// Loss: mean over (t = 1..T, movie, readout unit) of (clip(v_t + b) - target_t)^2.
+//
+// Layout: (units, 32) "fly-minor" with 32 movies; a wavefront works on two rows, one per half-wave (lane >> 5), in a
+// longest-row-first order so the two rows of a wavefront have similar lengths.  Every product uses the chunked order
+// of csrc/spmv.hip (16-entry chunks folded in order, chunk sums folded in order), every reduction a fixed order, so
+// forward and backward are deterministic and a recomputed segment reproduces the first forward bit for bit.
+#include "common.h"
+
+namespace {
+
+constexpr int NL = 32;   // movies per row: one half-wavefront
+
+__device__ __forceinline__ int half_row(const int* sched, int rows, int pair) {
+    const int k = 2 * pair + (threadIdx.x & 63) / NL;
+    return k < rows ? sched[k] : -1;
+}
+
+// sum over the 32 lanes of a half-wave in a fixed butterfly order (the same value on every lane of the half)
+__device__ __forceinline__ float half_sum(float x) {
+    x = x + __shfl_xor(x, 16, 32);
+    x = x + __shfl_xor(x, 8, 32);
+    x = x + __shfl_xor(x, 4, 32);
+    x = x + __shfl_xor(x, 2, 32);
+    x = x + __shfl_xor(x, 1, 32);
+    return x;
+}
+
+// chunked fold of row entries [e0, e1) of w * x[col, lane]
+__device__ __forceinline__ float fold32(const int* __restrict__ idx, const float* __restrict__ w,
+                                        const float* __restrict__ x, int e0, int e1, int l) {
+    float total = 0.0f;
+    for (int cs = e0; cs < e1; cs += 16) {
+        const int n = min(16, e1 - cs);
+        int c[16];
+        float wv[16], xv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < n) { c[j] = idx[cs + j]; wv[j] = w[cs + j]; }
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < n) xv[j] = x[(size_t)c[j] * NL + l];
+        }
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < n) { float p = wv[j] * xv[j]; s = s + p; }
+        }
+        total = total + s;
+    }
+    return total;
+}
+
+// Per-unit coefficients and per-edge weights from the parameters.
+__global__ __launch_bounds__(256) void k_adj_params(const float* __restrict__ tau, const float* __restrict__ bias,
+                                                    const int* __restrict__ unit_type, float* __restrict__ a,
+                                                    float* __restrict__ b, int n, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = unit_type[i];
+    a[i] = expf(-dt / tau[k]);
+    b[i] = bias[k];
+}
+
+__global__ __launch_bounds__(256) void k_adj_weights(const float* __restrict__ gain, const int* __restrict__ pair_bin,
+                                                     const float* __restrict__ w0, float* __restrict__ w, int64_t nnz) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    w[e] = gain[pair_bin[e]] * w0[e];
+}
+
+__global__ __launch_bounds__(256) void k_adj_dr(const float* __restrict__ v, const float* __restrict__ b,
+                                                float* __restrict__ dr, int64_t total) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const float base = b[t / NL];
+    float s = v[t] + base;
+    dr[t] = fh_clip01(s) - base;
+}
+
+struct FwdArgs {
+    const int *ptr, *idx, *sched;
+    const float *w, *a, *b;
+    const float *v, *adapt, *dr, *P;     // step t
+    float *v1, *adapt1, *dr1, *inp;      // step t+1 (inp of step t, may be null)
+    const int* readout;                  // (n) readout slot or -1
+    const float* target;                 // (R, 32) targets of step t+1, or null
+    float* loss;                         // (R) squared-error sums of step t+1 over the movies, or null
+    int rows;
+    float g_rr, g_ad, a_ad;
+};
+
+__global__ __launch_bounds__(256) void k_adj_forward(FwdArgs f) {
+    const int l = threadIdx.x & (NL - 1);
+    const int pairs = (f.rows + 1) / 2;
+    for (int pr = blockIdx.x * 4 + (threadIdx.x >> 6); pr < pairs; pr += gridDim.x * 4) {
+        const int r = half_row(f.sched, f.rows, pr);
+        if (r < 0) continue;
+        const float y = fold32(f.idx, f.w, f.dr, f.ptr[r], f.ptr[r + 1], l);
+        const int64_t q = (int64_t)r * NL + l;
+        float inp = f.g_rr * y;
+        inp = inp + f.P[q];
+        float gad = f.g_ad * f.adapt[q];
+        inp = inp - gad;
+        const float v = f.v[q];
+        float d = v - inp;
+        d = d * f.a[r];
+        const float vn = inp + d;
+        const float old = f.dr[q];
+        float da = f.adapt[q] - old;
+        da = da * f.a_ad;
+        f.v1[q] = vn;
+        f.adapt1[q] = old + da;
+        const float base = f.b[r];
+        float s = vn + base;
+        const float rn = fh_clip01(s);
+        f.dr1[q] = rn - base;
+        if (f.inp) f.inp[q] = inp;
+        if (f.loss) {
+            const int k = f.readout[r];
+            if (k >= 0) {       // the whole half-wave takes this branch together (same row)
+                float e = rn - f.target[(int64_t)k * NL + l];
+                e = e * e;
+                e = half_sum(e);
+                if (l == 0) f.loss[k] = e;
+            }
+        }
+    }
+}
+
+// Backward, part 1 (elementwise): ly = g_rr * lv * (1 - a); ga[i] += sum_b lv * (v_t - inp_t).
+__global__ __launch_bounds__(256) void k_adj_back_ly(const float* __restrict__ lv, const float* __restrict__ v,
+                                                     const float* __restrict__ inp, const float* __restrict__ a,
+                                                     float* __restrict__ ly, float* __restrict__ ga, int rows,
+                                                     float g_rr) {
+    const int l = threadIdx.x & (NL - 1);
+    const int r = blockIdx.x * (blockDim.x / NL) + (int)(threadIdx.x / NL);
+    if (r >= rows) return;
+    const int64_t q = (int64_t)r * NL + l;
+    const float lam = lv[q];
+    float oma = 1.0f - a[r];
+    float li = lam * oma;
+    ly[q] = g_rr * li;
+    float dv = v[q] - inp[q];
+    float c = lam * dv;
+    c = half_sum(c);
+    if (l == 0) ga[r] = ga[r] + c;
+}
+
+struct BackArgs {
+    const int *ptrT, *idxT, *schedT;
+    const float *wT, *a, *b;
+    const float *ly, *v, *dr;            // step t
+    float *lv, *la;                      // in: step t+1, out: step t (own rows only)
+    const int* readout;
+    const float* target;                 // (R, 32) targets of step t, or null (t = 0)
+    float* gb;                           // (n) per-unit bias gradient accumulator
+    int rows;
+    float g_ad, a_ad, loss_scale;
+};
+
+// Backward, part 2 (transpose product + the rest of the step): for unit j
+//   ldr = (W^T ly)_j + la_t+1 * (1 - a_ad)
+//   linp = lv_t+1 * (1 - a_j);  ladapt_t = la_t+1 * a_ad - g_ad * linp
+//   gate = [0 <= v_t + b_j <= 1] (torch clamp convention); lr = loss_scale * (r_t - target_t) on readout units
+//   lv_t = lv_t+1 * a_j + (ldr + lr) * gate;  gb_j += sum_b ldr * (gate - 1) + lr * gate
+__global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
+    const int l = threadIdx.x & (NL - 1);
+    const int pairs = (k.rows + 1) / 2;
+    for (int pr = blockIdx.x * 4 + (threadIdx.x >> 6); pr < pairs; pr += gridDim.x * 4) {
+        const int j = half_row(k.schedT, k.rows, pr);
+        if (j < 0) continue;
+        const float s = fold32(k.idxT, k.wT, k.ly, k.ptrT[j], k.ptrT[j + 1], l);
+        const int64_t q = (int64_t)j * NL + l;
+        const float la1 = k.la[q], lv1 = k.lv[q];
+        float c = la1 * (1.0f - k.a_ad);
+        const float ldr = s + c;
+        float linp = lv1 * (1.0f - k.a[j]);
+        float lad = la1 * k.a_ad;
+        float gl = k.g_ad * linp;
+        lad = lad - gl;
+        const float u = k.v[q] + k.b[j];
+        const float gate = (u >= 0.0f && u <= 1.0f) ? 1.0f : 0.0f;
+        float lr = 0.0f;
+        if (k.target) {
+            const int ro = k.readout[j];
+            if (ro >= 0) {
+                float diff = fh_clip01(u) - k.target[(int64_t)ro * NL + l];
+                lr = k.loss_scale * diff;
+            }
+        }
+        float lvn = lv1 * k.a[j];
+        float up = ldr + lr;
+        up = up * gate;
+        lvn = lvn + up;
+        k.lv[q] = lvn;
+        k.la[q] = lad;
+        float gbb = ldr * (gate - 1.0f);
+        float gr = lr * gate;
+        gbb = gbb + gr;
+        gbb = half_sum(gbb);
+        if (l == 0) k.gb[j] = k.gb[j] + gbb;
+    }
+}
+
+// Backward, part 3 (per edge of W): eg[e] += sum over movies of ly[i, b] * dr_t[col_e, b], lanes over edges of a row,
+// the 32 movies in order.
+__global__ __launch_bounds__(256) void k_adj_back_edges(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                        const float* __restrict__ ly, const float* __restrict__ dr,
+                                                        float* __restrict__ eg, int rows) {
+    const int lane = threadIdx.x & 63;
+    for (int r = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); r < rows; r += gridDim.x * 4) {
+        const int e0 = ptr[r], e1 = ptr[r + 1];
+        if (e0 == e1) continue;
+        float lyr[NL];
+        const float4* ly4 = (const float4*)(ly + (size_t)r * NL);
+#pragma unroll
+        for (int k = 0; k < NL / 4; ++k) {
+            float4 t = ly4[k];
+            lyr[4 * k] = t.x; lyr[4 * k + 1] = t.y; lyr[4 * k + 2] = t.z; lyr[4 * k + 3] = t.w;
+        }
+        for (int e = e0 + lane; e < e1; e += 64) {
+            const float4* d4 = (const float4*)(dr + (size_t)idx[e] * NL);
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NL / 4; ++k) {
+                float4 t = d4[k];
+                float p0 = lyr[4 * k] * t.x, p1 = lyr[4 * k + 1] * t.y, p2 = lyr[4 * k + 2] * t.z,
+                      p3 = lyr[4 * k + 3] * t.w;
+                s = s + p0; s = s + p1; s = s + p2; s = s + p3;
+            }
+            eg[e] = eg[e] + s;
+        }
+    }
+}
+
+// out[k] = scale_k * sum over the segment [seg[k], seg[k+1]) of vals[perm[m]] * (mult ? mult[perm[m]] : 1):
+// one wavefront per segment, lane-strided partial sums folded in lane order.
+__global__ __launch_bounds__(256) void k_segment_sum(const float* __restrict__ vals, const float* __restrict__ mult,
+                                                     const int* __restrict__ perm, const int* __restrict__ seg,
+                                                     float* __restrict__ out, int nseg) {
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= nseg) return;
+    float s = 0.0f;
+    for (int m = seg[k] + lane; m < seg[k + 1]; m += 64) {
+        const int e = perm[m];
+        float v = vals[e];
+        if (mult) v = v * mult[e];
+        s = s + v;
+    }
+    float total = 0.0f;
+    for (int j = 0; j < 64; ++j) total = total + fh_readlane(s, j);
+    if (lane == 0) out[k] = total;
+}
+
+// loss = sum over all (step, readout) partial sums, in order, times scale.
+__global__ void k_loss_total(const float* __restrict__ parts, int64_t n, float scale, float* __restrict__ out) {
+    const int lane = threadIdx.x;
+    float s = 0.0f;
+    for (int64_t m = lane; m < n; m += 64) s = s + parts[m];
+    float total = 0.0f;
+    for (int j = 0; j < 64; ++j) total = total + fh_readlane(s, j);
+    if (lane == 0) out[0] = total * scale;
+}
+
+inline unsigned nblk(int64_t total, int per = 256) { return (unsigned)((total + per - 1) / per); }
+
+}  // namespace
+
+// p = [tau (types), bias (types), unit_type (n), a (n), b (n)]
+FH_EXPORT int launch_adj_params(void** p, int n, float dt, void* stream) {
+    if (n < 0) return FH_ERR_ARGS;
+    if (!n) return 0;
+    hipLaunchKernelGGL(k_adj_params, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
+                       (const float*)p[1], (const int*)p[2], (float*)p[3], (float*)p[4], n, dt);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [gain (types*types), pair_bin (nnz), w0 (nnz), w (nnz)]
+FH_EXPORT int launch_adj_weights(void** p, int64_t nnz, void* stream) {
+    if (nnz < 0) return FH_ERR_ARGS;
+    if (!nnz) return 0;
+    hipLaunchKernelGGL(k_adj_weights, dim3(nblk(nnz)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
+                       (const int*)p[1], (const float*)p[2], (float*)p[3], nnz);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [v (n, 32), b (n), dr (n, 32)]
+FH_EXPORT int launch_adj_dr(void** p, int n, void* stream) {
+    if (n < 0) return FH_ERR_ARGS;
+    if (!n) return 0;
+    hipLaunchKernelGGL(k_adj_dr, dim3(nblk((int64_t)n * NL)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
+                       (const float*)p[1], (float*)p[2], (int64_t)n * NL);
+    FH_LAUNCH_CHECK();
+}
+
+// One forward step.  p = [ptr, idx, sched, w, a, b, v, adapt, dr, P, v1, adapt1, dr1, inp|null, readout,
+// target|null, loss|null]; f = [g_rr, g_ad, a_ad].
+FH_EXPORT int launch_adj_forward(void** p, int rows, const float* f, int grid, void* stream) {
+    if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
+    if (!rows) return 0;
+    FwdArgs a;
+    a.ptr = (const int*)p[0]; a.idx = (const int*)p[1]; a.sched = (const int*)p[2];
+    a.w = (const float*)p[3]; a.a = (const float*)p[4]; a.b = (const float*)p[5];
+    a.v = (const float*)p[6]; a.adapt = (const float*)p[7]; a.dr = (const float*)p[8]; a.P = (const float*)p[9];
+    a.v1 = (float*)p[10]; a.adapt1 = (float*)p[11]; a.dr1 = (float*)p[12]; a.inp = (float*)p[13];
+    a.readout = (const int*)p[14]; a.target = (const float*)p[15]; a.loss = (float*)p[16];
+    a.rows = rows; a.g_rr = f[0]; a.g_ad = f[1]; a.a_ad = f[2];
+    if (a.loss && !a.target) return FH_ERR_ARGS;
+    hipLaunchKernelGGL(k_adj_forward, dim3(grid), dim3(256), 0, (hipStream_t)stream, a);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [lv, v, inp, a, ly, ga]
+FH_EXPORT int launch_adj_back_ly(void** p, int rows, float g_rr, void* stream) {
+    if (rows < 0) return FH_ERR_ARGS;
+    if (!rows) return 0;
+    hipLaunchKernelGGL(k_adj_back_ly, dim3(nblk((int64_t)rows * NL)), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)p[0], (const float*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4],
+                       (float*)p[5], rows, g_rr);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [ptrT, idxT, schedT, wT, a, b, ly, v, dr, lv, la, readout, target|null, gb]; f = [g_ad, a_ad, loss_scale].
+FH_EXPORT int launch_adj_back_rows(void** p, int rows, const float* f, int grid, void* stream) {
+    if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
+    if (!rows) return 0;
+    BackArgs k;
+    k.ptrT = (const int*)p[0]; k.idxT = (const int*)p[1]; k.schedT = (const int*)p[2];
+    k.wT = (const float*)p[3]; k.a = (const float*)p[4]; k.b = (const float*)p[5];
+    k.ly = (const float*)p[6]; k.v = (const float*)p[7]; k.dr = (const float*)p[8];
+    k.lv = (float*)p[9]; k.la = (float*)p[10]; k.readout = (const int*)p[11]; k.target = (const float*)p[12];
+    k.gb = (float*)p[13];
+    k.rows = rows; k.g_ad = f[0]; k.a_ad = f[1]; k.loss_scale = f[2];
+    hipLaunchKernelGGL(k_adj_back_rows, dim3(grid), dim3(256), 0, (hipStream_t)stream, k);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [ptr, idx, ly, dr, eg]
+FH_EXPORT int launch_adj_back_edges(void** p, int rows, int grid, void* stream) {
+    if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
+    if (!rows) return 0;
+    hipLaunchKernelGGL(k_adj_back_edges, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
+                       (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], rows);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [vals, mult|null, perm, seg (nseg+1), out]
+FH_EXPORT int launch_segment_sum(void** p, int nseg, void* stream) {
+    if (nseg < 0) return FH_ERR_ARGS;
+    if (!nseg) return 0;
+    hipLaunchKernelGGL(k_segment_sum, dim3(nblk(nseg, 4)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
+                       (const float*)p[1], (const int*)p[2], (const int*)p[3], (float*)p[4], nseg);
+    FH_LAUNCH_CHECK();
+}
+
+// p = [parts, out]
+FH_EXPORT int launch_loss_total(void** p, int64_t n, float scale, void* stream) {
+    if (n < 0) return FH_ERR_ARGS;
+    hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(64), 0, (hipStream_t)stream, (const float*)p[0], n, scale,
+                       (float*)p[1]);
+    FH_LAUNCH_CHECK();
+}

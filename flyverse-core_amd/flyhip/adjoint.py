@@ -1,0 +1,245 @@
+"""Differentiable optic lobe with a hand-written adjoint (an addition: the reference differentiates its optic lobe
+only through torch autograd, flyverse/optic.py:606-676, and has no adjoint or checkpointing).
+
+The rate network of flyverse/optic.py:525-551 without spiking feedback, on 32 movies at once, with per-type time
+constants tau, per-type operating points (biases) b and a gain per (postsynaptic type, presynaptic type) pair on the
+normalised recurrent weights (the parameters OpticParams.tau_by_type / baseline_by_type / pair_gain expose,
+optic.py:82-87, 251-267, 343-347):
+
+    dr_t = clip(v_t + b, 0, 1) - b;  y_t = (G o W0) dr_t;  inp_t = g_rr y_t + P_t - g_ad adapt_t
+    v_t+1 = inp_t + (v_t - inp_t) a,  a = exp(-dt / tau);  adapt_t+1 = dr_t + (adapt_t - dr_t) a_ad
+    loss = mean over t = 1..T, movies and readout units of (clip(v_t + b, 0, 1) - target_t)^2
+
+`OpticAdjoint.loss(tau, bias, gain)` runs the forward (FP32, csrc/adjoint.hip) keeping (v, adapt) every
+`checkpoint` steps; `gradients()` walks the segments backwards, recomputing each segment's states from its
+checkpoint and running the adjoint step by step: the transpose product (W^T applied to g_rr lambda_inp through the
+transposed CSR), the clamp gates (gradient passes where 0 <= v + b <= 1, torch's convention for clamp), and a
+per-edge accumulation of sum_b lambda_y dr for the pair gains.  Per-type and per-pair sums are fixed-order segmented
+reductions, so the gradient is deterministic.  `OpticLossFn` wraps both in a torch.autograd.Function.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import hip
+
+NL = 32
+_P, _I, _F, _S, _L = ct.POINTER(ct.c_void_p), ct.c_int, ct.c_float, ct.c_void_p, ct.c_int64
+_FP = ct.POINTER(ct.c_float)
+SIGS = {
+    "launch_adj_params": [_P, _I, _F, _S],
+    "launch_adj_weights": [_P, _L, _S],
+    "launch_adj_dr": [_P, _I, _S],
+    "launch_adj_forward": [_P, _I, _FP, _I, _S],
+    "launch_adj_back_ly": [_P, _I, _F, _S],
+    "launch_adj_back_rows": [_P, _I, _FP, _I, _S],
+    "launch_adj_back_edges": [_P, _I, _I, _S],
+    "launch_segment_sum": [_P, _I, _S],
+    "launch_loss_total": [_P, _L, _F, _S],
+}
+
+
+def _lib(device):
+    L = hip.lib(device)
+    if not getattr(L, "_adj_sigs", False):
+        for name, sig in SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = sig
+            fn.restype = _I
+        L._adj_sigs = True
+    return L
+
+
+def _segments(keys: np.ndarray, nseg: int):
+    perm = np.argsort(keys, kind="stable").astype(np.int32)
+    seg = np.zeros(nseg + 1, np.int64)
+    np.cumsum(np.bincount(keys, minlength=nseg), out=seg[1:])
+    return perm, seg.astype(np.int32)
+
+
+@dataclass
+class OpticConstants:
+    dt: float = 5.0
+    gain_rr: float = 1.0
+    adapt_gain: float = 1.0
+    adapt_tau: float = 400.0
+
+
+class OpticAdjoint:
+    """The C4-style training problem on the GPU.  W0: (n, n) normalised signed weights (post, pre); unit_type (n,)
+    in [0, K); P: (T, n, 32) held input drive per step; target: (T, R, 32) for steps 1..T; readout: (R,) units."""
+
+    def __init__(self, W0: sp.spmatrix, unit_type, n_types: int, P, target, readout, consts=None, checkpoint=20,
+                 device="cuda", grid=None):
+        self.device = dev = torch.device(device)
+        if not hip.is_rocm_gpu(dev):
+            raise ValueError("the optic adjoint runs on a ROCm GPU only")
+        _lib(dev)
+        c = self.c = consts or OpticConstants()
+        W0 = sp.csr_matrix(W0, dtype=np.float32)
+        W0.sum_duplicates()
+        W0.sort_indices()
+        self.n = n = W0.shape[0]
+        self.K = K = int(n_types)
+        ut = np.asarray(unit_type, np.int64)
+        self.T = int(P.shape[0])
+        if tuple(P.shape) != (self.T, n, NL):
+            raise ValueError(f"P must be (T, n, {NL})")
+        self.R = len(readout)
+        if tuple(target.shape) != (self.T, self.R, NL):
+            raise ValueError(f"target must be (T, R, {NL})")
+        self.checkpoint = int(checkpoint)
+        t = lambda a, dt=None: torch.as_tensor(np.ascontiguousarray(a), device=dev, dtype=dt)
+        rows = np.repeat(np.arange(n), np.diff(W0.indptr))
+        self.ptr, self.idx = t(W0.indptr.astype(np.int32)), t(W0.indices.astype(np.int32))
+        self.w0 = t(W0.data)
+        bins = (ut[rows] * K + ut[W0.indices]).astype(np.int32)
+        self.bin = t(bins)
+        self.sched = t(np.argsort(-np.diff(W0.indptr), kind="stable").astype(np.int32))
+        WT = W0.T.tocsr()
+        WT.sort_indices()
+        rowsT = np.repeat(np.arange(n), np.diff(WT.indptr))
+        self.ptrT, self.idxT = t(WT.indptr.astype(np.int32)), t(WT.indices.astype(np.int32))
+        self.w0T = t(WT.data.astype(np.float32))
+        self.binT = t((ut[WT.indices] * K + ut[rowsT]).astype(np.int32))
+        self.schedT = t(np.argsort(-np.diff(WT.indptr), kind="stable").astype(np.int32))
+        self.nnz = W0.nnz
+        pg, sg = _segments(bins.astype(np.int64), K * K)
+        self.permG, self.segG = t(pg), t(sg)
+        pt, st = _segments(ut, K)
+        self.permT, self.segT = t(pt), t(st)
+        self.unit_type = t(ut.astype(np.int32))
+        ro = -np.ones(n, np.int32)
+        ro[np.asarray(readout)] = np.arange(self.R, dtype=np.int32)
+        self.readout = t(ro)
+        self.P = torch.as_tensor(P, dtype=torch.float32, device=dev).contiguous()
+        self.target = torch.as_tensor(target, dtype=torch.float32, device=dev).contiguous()
+        z = lambda *s: torch.zeros(*s, device=dev)
+        self.a, self.b = z(n), z(n)
+        self.w, self.wT = z(self.nnz), z(self.nnz)
+        self.a_ad = float(math.exp(-c.dt / c.adapt_tau))
+        self.f_fwd = (ct.c_float * 3)(c.gain_rr, c.adapt_gain, self.a_ad)
+        self.loss_parts = z(self.T, self.R)
+        self.loss_out = z(1)
+        self.grid = grid or torch.cuda.get_device_properties(dev).multi_processor_count * 16
+        nseg = (self.T + self.checkpoint - 1) // self.checkpoint
+        self.ck_v = z(nseg, n, NL)
+        self.ck_a = z(nseg, n, NL)
+        C = self.checkpoint
+        self.S_v, self.S_a, self.S_dr, self.S_inp = z(C + 1, n, NL), z(C + 1, n, NL), z(C + 1, n, NL), z(C, n, NL)
+        self.lv, self.la, self.ly = z(n, NL), z(n, NL), z(n, NL)
+        self.ga, self.gb, self.eg = z(n), z(n), z(self.nnz)
+        self._params = None
+
+    # ------------------------------------------------------------------ launches
+    def _call(self, name, tensors, *args):
+        arr = (ct.c_void_p * len(tensors))(*(None if x is None else x.data_ptr() for x in tensors))
+        err = getattr(_lib(self.device), name)(arr, *args, torch.cuda.current_stream(self.device).cuda_stream)
+        if err:
+            raise RuntimeError(f"{name}: HIP error {err}")
+
+    def _set_params(self, tau, bias, gain):
+        tau = tau.detach().to(self.device, torch.float32).contiguous()
+        bias = bias.detach().to(self.device, torch.float32).contiguous()
+        gain = gain.detach().to(self.device, torch.float32).reshape(-1).contiguous()
+        if tau.numel() != self.K or bias.numel() != self.K or gain.numel() != self.K * self.K:
+            raise ValueError("tau and bias need one value per type, gain one per type pair")
+        self._params = (tau, bias, gain)
+        self._call("launch_adj_params", [tau, bias, self.unit_type, self.a, self.b], self.n, float(self.c.dt))
+        self._call("launch_adj_weights", [gain, self.bin, self.w0, self.w], self.nnz)
+        self._call("launch_adj_weights", [gain, self.binT, self.w0T, self.wT], self.nnz)
+
+    def _step(self, t, v, ad, dr, v1, ad1, dr1, inp=None):
+        self._call("launch_adj_forward",
+                   [self.ptr, self.idx, self.sched, self.w, self.a, self.b, v, ad, dr, self.P[t], v1, ad1, dr1, inp,
+                    self.readout, self.target[t], self.loss_parts[t]],
+                   self.n, self.f_fwd, self.grid)
+
+    # ------------------------------------------------------------------ forward
+    def loss(self, tau, bias, gain) -> torch.Tensor:
+        """Forward over T steps; returns the loss (a 1-element device tensor) and keeps the checkpoints."""
+        self._set_params(tau, bias, gain)
+        n = self.n
+        v = [torch.zeros(n, NL, device=self.device) for _ in range(2)]
+        ad = [torch.zeros(n, NL, device=self.device) for _ in range(2)]
+        dr = [torch.empty(n, NL, device=self.device) for _ in range(2)]
+        self._call("launch_adj_dr", [v[0], self.b, dr[0]], n)
+        for t in range(self.T):
+            cur, nxt = t % 2, (t + 1) % 2
+            if t % self.checkpoint == 0:
+                self.ck_v[t // self.checkpoint].copy_(v[cur])
+                self.ck_a[t // self.checkpoint].copy_(ad[cur])
+            self._step(t, v[cur], ad[cur], dr[cur], v[nxt], ad[nxt], dr[nxt])
+        self._call("launch_loss_total", [self.loss_parts, self.loss_out], self.T * self.R,
+                   1.0 / (self.T * NL * self.R))
+        return self.loss_out
+
+    # ------------------------------------------------------------------ backward
+    def gradients(self):
+        """(d loss/d tau, d loss/d bias, d loss/d gain) at the parameters of the last `loss` call."""
+        if self._params is None:
+            raise RuntimeError("call loss() first")
+        n, C, T = self.n, self.checkpoint, self.T
+        scale = 2.0 / (T * NL * self.R)
+        f_back = (ct.c_float * 3)(self.c.adapt_gain, self.a_ad, scale)
+        for buf in (self.lv, self.la, self.ga, self.gb, self.eg):
+            buf.zero_()
+        nseg = (T + C - 1) // C
+        first = True
+        for s in reversed(range(nseg)):
+            t0, t1 = s * C, min(T, (s + 1) * C)
+            # recompute the segment's states from its checkpoint (the same launches as the forward: same bits)
+            self.S_v[0].copy_(self.ck_v[s])
+            self.S_a[0].copy_(self.ck_a[s])
+            self._call("launch_adj_dr", [self.S_v[0], self.b, self.S_dr[0]], n)
+            for k in range(t1 - t0):
+                self._step(t0 + k, self.S_v[k], self.S_a[k], self.S_dr[k], self.S_v[k + 1], self.S_a[k + 1],
+                           self.S_dr[k + 1], self.S_inp[k])
+            if first:
+                # the loss term of the last step, T: lambda_v_T = loss_scale (r_T - target_T) gate_T
+                self.ly.zero_()
+                self._back_rows(t1 - t0, T, f_back)
+                first = False
+            for k in reversed(range(t1 - t0)):
+                t = t0 + k
+                self._call("launch_adj_back_ly", [self.lv, self.S_v[k], self.S_inp[k], self.a, self.ly, self.ga], n,
+                           float(self.c.gain_rr))
+                self._back_rows(k, t, f_back)
+                self._call("launch_adj_back_edges", [self.ptr, self.idx, self.ly, self.S_dr[k], self.eg], n, self.grid)
+        tau, bias, gain = self._params
+        K = self.K
+        ga_t, gb_t, gg = (torch.empty(K, device=self.device), torch.empty(K, device=self.device),
+                          torch.empty(K * K, device=self.device))
+        self._call("launch_segment_sum", [self.ga, None, self.permT, self.segT, ga_t], K)
+        self._call("launch_segment_sum", [self.gb, None, self.permT, self.segT, gb_t], K)
+        self._call("launch_segment_sum", [self.eg, self.w0, self.permG, self.segG, gg], K * K)
+        a_k = torch.exp(-self.c.dt / tau)
+        g_tau = ga_t * a_k * (self.c.dt / (tau * tau))
+        return g_tau, gb_t, gg.view(K, K)
+
+    def _back_rows(self, k, t, f_back):
+        target = self.target[t - 1] if t >= 1 else None
+        self._call("launch_adj_back_rows",
+                   [self.ptrT, self.idxT, self.schedT, self.wT, self.a, self.b, self.ly, self.S_v[k], self.S_dr[k],
+                    self.lv, self.la, self.readout, target, self.gb],
+                   self.n, f_back, self.grid)
+
+
+class OpticLossFn(torch.autograd.Function):
+    """loss = OpticLossFn.apply(tau, bias, gain, problem): forward and the hand-written adjoint as one autograd node."""
+
+    @staticmethod
+    def forward(ctx, tau, bias, gain, problem: OpticAdjoint):
+        ctx.problem = problem
+        return problem.loss(tau, bias, gain).clone().reshape(())
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        g_tau, g_b, g_g = ctx.problem.gradients()
+        return grad_out * g_tau, grad_out * g_b, grad_out * g_g, None
