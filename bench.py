@@ -105,6 +105,41 @@ def time_lif_launches(bb, n=6):
     return float(np.mean(times[1:])) if len(times) > 1 else times[0]
 
 
+def optic_training(steps=3, warmup=1):
+    """The differentiable optic lobe at full scale: forward + hand-written adjoint per training step."""
+    import torch
+    from flyhip import optic_data as od
+    from flyhip.adjoint import OpticAdjoint
+    W0, ut, W_rp, pos = od.optic_network()
+    inten = od.grating_intensity(pos, od.T_STEPS)
+    P = od.photoreceptor_drive(inten, W_rp)
+    rng = np.random.default_rng(8)
+    readout = np.sort(rng.choice(W0.shape[0], od.N_READOUT, replace=False))
+    tau, bias, gain = od.initial_params()
+    prob = OpticAdjoint(W0, ut, od.N_TYPES, P, np.zeros((od.T_STEPS, od.N_READOUT, 32), np.float32), readout)
+    del P
+    t = lambda x: torch.as_tensor(x, device="cuda")
+    teacher = od.teacher_params(tau, bias, gain)
+    prob.target.copy_(prob.readout_trajectory(*[t(x) for x in teacher]))
+    th = [t(x) for x in (tau, bias, gain)]
+    for _ in range(warmup):
+        prob.loss(*th)
+        prob.gradients()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        L = prob.loss(*th)
+        g = prob.gradients()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    gn = float(torch.sqrt(sum((x.double() ** 2).sum() for x in g)))
+    return {"ol_fwd_bwd_ms_per_step": round(ms, 3), "ol_us_per_simulated_step": round(ms * 1000 / od.T_STEPS, 2),
+            "ol_units": int(W0.shape[0]), "ol_edges": int(W0.nnz), "ol_movies": 32, "ol_steps": od.T_STEPS,
+            "ol_checkpoint": prob.checkpoint, "ol_loss": float(L), "ol_grad_norm": gn}
+
+
 def cpu_baseline(conn, retina, lp, op, rollout, frames):
     """The CPU oracle (plain C, one thread) running one fly of the workload for `frames` frames."""
     sys.path.insert(0, str(ROOT))
@@ -127,6 +162,7 @@ def main():
     ap.add_argument("--b1-frames", type=int, default=30)
     ap.add_argument("--cpu-frames", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-optic-training", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -200,6 +236,14 @@ def main():
     del fb
     torch.cuda.empty_cache()
 
+    ol = None
+    if not args.no_optic_training:
+        log("optic-lobe training step (forward + adjoint), full scale ...")
+        ol = optic_training()
+        log(f"OL fwd+bwd: {ol['ol_fwd_bwd_ms_per_step']} ms per training step "
+            f"({ol['ol_us_per_simulated_step']} us per simulated step)")
+        torch.cuda.empty_cache()
+
     if args.copy_calibration:
         x = torch.empty(2**28, dtype=torch.float32, device=dev)
         y = torch.empty_like(x)
@@ -229,7 +273,8 @@ def main():
                                "10 ms frame",
                    "batch": 64, "neurons": int(conn.n), "edges": int(conn.W.nnz), "dt_ms": lp.dt,
                    "b1_sim_s_per_wall_s": round(b1, 4), "ratio_b64_vs_b1": round(value / b1, 2),
-                   "b1_ms_per_frame": round(b1_ms / args.b1_frames, 4), "spike_activity_per_step": activity},
+                   "b1_ms_per_frame": round(b1_ms / args.b1_frames, 4), "spike_activity_per_step": activity,
+                   **(ol or {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_batch_lif", "launch_ms": round(lif_ms, 4), "steps_per_launch": K,

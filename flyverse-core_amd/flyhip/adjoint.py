@@ -180,6 +180,25 @@ class OpticAdjoint:
                    1.0 / (self.T * NL * self.R))
         return self.loss_out
 
+    def readout_trajectory(self, tau, bias, gain) -> torch.Tensor:
+        """(T, R, 32) readout rates clip(v_t + b, 0, 1) for t = 1..T of a forward run (used to make teacher
+        targets); the loss computed alongside is against the current targets and is discarded."""
+        self._set_params(tau, bias, gain)
+        n = self.n
+        ro = torch.nonzero(self.readout >= 0).flatten()
+        order = torch.argsort(self.readout[ro])
+        ro = ro[order]
+        v = [torch.zeros(n, NL, device=self.device) for _ in range(2)]
+        ad = [torch.zeros(n, NL, device=self.device) for _ in range(2)]
+        dr = [torch.empty(n, NL, device=self.device) for _ in range(2)]
+        self._call("launch_adj_dr", [v[0], self.b, dr[0]], n)
+        out = torch.empty(self.T, self.R, NL, device=self.device)
+        for t in range(self.T):
+            cur, nxt = t % 2, (t + 1) % 2
+            self._step(t, v[cur], ad[cur], dr[cur], v[nxt], ad[nxt], dr[nxt])
+            out[t] = (v[nxt][ro] + self.b[ro][:, None]).clamp(0.0, 1.0)
+        return out
+
     # ------------------------------------------------------------------ backward
     def gradients(self):
         """(d loss/d tau, d loss/d bias, d loss/d gain) at the parameters of the last `loss` call."""
