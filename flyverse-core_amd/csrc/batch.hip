@@ -108,34 +108,54 @@ __device__ __forceinline__ uint32_t transpose32(uint32_t x, int lane) {
 // fly b's copy of the presynaptic cell fired); after the transpose lane b holds the bits of its own fly over the 64
 // edges, and walks its set bits in ascending edge order adding the staged weights.  The work per chunk is the
 // transpose plus as many rounds as the busiest lane has set bits, instead of two lane reads per edge.
+__device__ __forceinline__ float fold_chunk_transposed(float acc, uint64_t mj, float wj, int lane, float* wlds) {
+    wlds[lane] = wj;
+    uint32_t lo = (uint32_t)mj, hi = (uint32_t)(mj >> 32);
+    // 64 x 64 = four 32 x 32 blocks: swap the off-diagonal halves between lane j and j ^ 32, then transpose each
+    const uint32_t send = lane < 32 ? hi : lo;
+    const uint32_t got = (uint32_t)__shfl_xor((int)send, 32);
+    if (lane < 32) hi = got; else lo = got;
+    lo = transpose32(lo, lane);
+    hi = transpose32(hi, lane);
+    __builtin_amdgcn_wave_barrier();
+    while (__builtin_amdgcn_ballot_w64((lo | hi) != 0u)) {
+        if ((lo | hi) != 0u) {
+            const int j = lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi);
+            acc = acc + wlds[j];
+            if (lo) lo &= lo - 1u; else hi &= hi - 1u;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return acc;
+}
+
+// The fold with the 64 x 64 bit matrix of a 64-edge chunk transposed: lane j gathers the mask of edge j (bit b =
+// fly b's copy of the presynaptic cell fired); after the transpose lane b holds the bits of its own fly over the 64
+// edges, and walks its set bits in ascending edge order adding the staged weights.  The work per chunk is the
+// transpose plus as many rounds as the busiest lane has set bits, instead of two lane reads per edge.  The gathers
+// of four chunks are issued together: they do not depend on the running sum, so only the adds form the chain.
 __device__ __forceinline__ float fold_edges_transposed(float acc, const uint32_t* __restrict__ off,
                                                        const float* __restrict__ w, const char* slot, int e0, int e1,
                                                        int lane, float* wlds) {
-    for (int cb = e0; cb < e1; cb += 64) {
-        const int e = cb + lane;
-        uint64_t mj = 0;
-        float wj = 0.0f;
-        if (e < e1) {
-            mj = *(const uint64_t*)(slot + off[e]);
-            wj = w[e];
-        }
-        wlds[lane] = wj;
-        uint32_t lo = (uint32_t)mj, hi = (uint32_t)(mj >> 32);
-        // 64 x 64 = four 32 x 32 blocks: swap the off-diagonal halves between lane j and j ^ 32, then transpose each
-        const uint32_t send = lane < 32 ? hi : lo;
-        const uint32_t got = (uint32_t)__shfl_xor((int)send, 32);
-        if (lane < 32) hi = got; else lo = got;
-        lo = transpose32(lo, lane);
-        hi = transpose32(hi, lane);
-        __builtin_amdgcn_wave_barrier();
-        while (__builtin_amdgcn_ballot_w64((lo | hi) != 0u)) {
-            if ((lo | hi) != 0u) {
-                const int j = lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi);
-                acc = acc + wlds[j];
-                if (lo) lo &= lo - 1u; else hi &= hi - 1u;
+    constexpr int U = 4;
+    for (int cb = e0; cb < e1; cb += U * 64) {
+        uint64_t mj[U];
+        float wj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = cb + u * 64 + lane;
+            mj[u] = 0;
+            wj[u] = 0.0f;
+            if (e < e1) {
+                mj[u] = *(const uint64_t*)(slot + off[e]);
+                wj[u] = w[e];
             }
         }
-        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (cb + u * 64 >= e1) break;
+            acc = fold_chunk_transposed(acc, mj[u], wj[u], lane, wlds);
+        }
     }
     return acc;
 }

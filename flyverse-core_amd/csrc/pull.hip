@@ -40,6 +40,68 @@ __global__ __launch_bounds__(256) void k_pull(const int* __restrict__ ptr, const
     if (lane == 0) g[gw] = acc;
 }
 
+// The pull over packed spike bits (transmitted values 0/1): the spike vector of a fly is n bits, small enough to stay
+// in the vector L1 cache, so the presynaptic lookups of all rows hit on-chip memory.  Rows are visited longest first
+// over a persistent grid; rows without in-edges only decay, 64 per wavefront.
+__global__ __launch_bounds__(256) void k_pull_bits(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                   const float* __restrict__ w, const uint64_t* __restrict__ bits,
+                                                   float* __restrict__ g, const int* __restrict__ sched, int n,
+                                                   int n_full, int batch, float a_s, int decay) {
+    const int lane = threadIdx.x & 63;
+    const int nw = (n + 63) >> 6;
+    const int64_t full_items = (int64_t)n_full * batch;
+    const int empty_blocks = (n - n_full + 63) >> 6;
+    const int64_t items = full_items + (int64_t)empty_blocks * batch;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t it = (int64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+         it < items; it += waves) {
+        if (it < full_items) {
+            const int b = (int)(it / n_full), r = sched[it % n_full];
+            const uint64_t* wb = bits + (int64_t)b * nw;
+            float* gp = g + (int64_t)b * n + r;
+            float acc = *gp;
+            if (decay) acc = acc * a_s;
+            const int e0 = ptr[r], e1 = ptr[r + 1];
+            // 512 in-edges per round: all index loads first, then all bit lookups, then the folds, so a wavefront
+            // waits for memory once per round instead of once per 64 edges (long rows are the critical path)
+            constexpr int U = 8;
+            for (int base = e0; base < e1; base += U * kWave) {
+                int pre[U];
+                bool act[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int e = base + u * kWave + lane;
+                    pre[u] = e < e1 ? idx[e] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    act[u] = pre[u] >= 0 && ((wb[pre[u] >> 6] >> (pre[u] & 63)) & 1ull);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (base + u * kWave >= e1) break;
+                    const int e = base + u * kWave + lane;
+                    const float c = act[u] ? w[e] : 0.0f;
+                    uint64_t m = __builtin_amdgcn_ballot_w64(act[u]);
+                    while (m) {
+                        const int j = __builtin_ctzll(m);
+                        acc = acc + fh_readlane(c, j);
+                        m &= m - 1;
+                    }
+                }
+            }
+            if (lane == 0) *gp = acc;
+        } else {
+            const int64_t it2 = it - full_items;
+            const int b = (int)(it2 / empty_blocks);
+            const int k = n_full + (int)(it2 % empty_blocks) * 64 + lane;
+            if (k < n && decay) {
+                float* gp = g + (int64_t)b * n + sched[k];
+                *gp = *gp * a_s;
+            }
+        }
+    }
+}
+
 // CSC-compatible deterministic scatter: the contract of launch_events (neural.cu:124) without atomics.  One
 // wavefront per (fly, tile of kTile posts) holds g's tile in LDS and walks the transmitting presynaptic cells in
 // ascending order; a cell's out-edges have distinct posts, so one pass over them needs no ordering, and the walk
@@ -107,6 +169,18 @@ FH_EXPORT int launch_pull(void** p, int n, int batch, float a_s, int flags, void
     else
         hipLaunchKernelGGL(k_pull<float>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
                            (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], n, batch, a_s, decay);
+    FH_LAUNCH_CHECK();
+}
+
+// launch_pull for 0/1 spikes given as packed bits.  p = [ptr (n+1), idx, w (float), bits uint64 (batch,
+// ceil(n/64)), g (batch, n), sched (n: the rows with in-edges first, longest first, then the rest)];
+// n_full = rows with in-edges; grid = workgroups of the persistent launch.
+FH_EXPORT int launch_pull_bits(void** p, int n, int n_full, int batch, float a_s, int flags, int grid, void* stream) {
+    if (n < 0 || batch < 0 || n_full < 0 || n_full > n || grid <= 0) return FH_ERR_ARGS;
+    if ((int64_t)n * batch == 0) return 0;
+    hipLaunchKernelGGL(k_pull_bits, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const int*)p[0], (const int*)p[1],
+                       (const float*)p[2], (const uint64_t*)p[3], (float*)p[4], (const int*)p[5], n, n_full, batch,
+                       a_s, (flags & 2) ? 0 : 1);
     FH_LAUNCH_CHECK();
 }
 

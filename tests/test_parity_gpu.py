@@ -320,3 +320,56 @@ def test_fly_frames_bit_exact_with_optic(graphs):
     for k in ("v", "g", "rate", "refrac"):
         assert_bits(N_(getattr(fb.brain, k)), getattr(om.st, k), k)
     assert om.st.spike_counts.sum() > 0
+
+
+def _pack_bits(x):
+    B, n = x.shape
+    nw = (n + 63) // 64
+    on = np.zeros((B, nw * 64), np.uint64)
+    on[:, :n] = (x != 0)
+    w = (on.reshape(B, nw, 64) << np.arange(64, dtype=np.uint64)).sum(-1, dtype=np.uint64)
+    return w.view(np.int64)
+
+
+@pytest.mark.parametrize("B,activity", [(1, 0.02), (3, 0.3)])
+def test_pull_bits_bit_exact(B, activity):
+    rng = np.random.default_rng(21)
+    n = 2100
+    W = random_csr(rng, n, n, 0.03, long_rows=(5,), long_len=1500)
+    W = sp.vstack([W[:1900], sp.csr_matrix((200, n), dtype=np.float32)]).tocsr()   # 200 rows without in-edges
+    W.data *= 5.0
+    x = (rng.random((B, n)) < activity).astype(np.float32)
+    g0 = rng.normal(0, 3, (B, n)).astype(np.float32)
+    Wc = _csc(W)
+    ref = g0.copy()
+    oracle.scale(ref, 0.98)
+    oracle.events(Wc.indptr, Wc.indices, Wc.data, x, ref)
+    deg = np.diff(W.indptr)
+    sched = np.argsort(-deg, kind="stable").astype(np.int32)
+    g = T(g0)
+    hip.pull_bits(T(W.indptr.astype(np.int32)), T(W.indices.astype(np.int32)), T(W.data), T(_pack_bits(x)), g,
+                  T(sched), int((deg > 0).sum()), 0.98, grid=64)
+    assert_bits(N_(g), ref, "pull_bits")
+
+
+def test_brain_with_depression_and_adaptation_bit_exact():
+    """STD and adaptation on (the float spike_buf pull path) against the oracle's brain_step, 300 steps."""
+    c, inputs, hz = synth.plain_lif_graph(6, n=2000, n_inputs=80)
+    p = LIFParams.shiu(dt=0.1, std_u=0.3, adapt_jump=1.2)
+    B = 2
+    brain = Brain(c, p, device=DEV, batch=B, seed=77)
+    brain.record_activity = True
+    brain.set_poisson(inputs, hz)
+    from helpers import oracle_fly
+    om = oracle_fly(c, None, p, None, B, 77, None, with_optic=False)
+    om.flags = 32 | 2 | 8
+    om.P = oracle.lif_params_vector(p)
+    om.st.std_u[:] = np.float32(p.std_u)
+    om.st.poisson_p[:, inputs] = (hz * np.float32(p.dt / 1000.0)).astype(np.float32)
+    for step, st in enumerate(om.lif_steps(300)):
+        brain.step()
+        assert_bits(N_(brain.spikes), st.spikes, f"spikes step {step}")
+    for k in ("v", "g", "refrac", "rate", "adapt", "res"):
+        assert_bits(N_(getattr(brain, k)), getattr(om.st, k), k)
+    assert_bits(N_(brain.spike_buf), om.st.spike_buf, "spike_buf")
+    assert om.st.spike_counts.sum() > 0
