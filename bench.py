@@ -68,15 +68,15 @@ def lif_block_bytes(bb, K, B):
     import torch
     T = int(bb.engine_step.item())
     n = min(bb.ring_slots, T)
-    ring = bb.ring.cpu().numpy().view(np.uint64)
+    slots = {t: bb.ring_slot(t) for t in range(T - n, T)}
     outdeg = np.bincount(bb.W_lif.indices, minlength=bb.n_rows).astype(np.float64)
     touched = []
     for t in range(T - n, T):
-        fired = ring[t % bb.ring_slots] != 0
+        fired = slots[t] != 0
         touched.append(outdeg[fired].sum())
     N = bb.n_rows
     per_step = B * N * (32 + 8 + 1 / 8) + 8 * float(np.mean(touched)) + B * N / 8
-    act = float(np.mean([np.unpackbits(ring[t % bb.ring_slots].view(np.uint8)).sum() for t in range(T - n, T)]))
+    act = float(np.mean([np.unpackbits(slots[t].view(np.uint8)).sum() for t in range(T - n, T)]))
     return K * per_step, act / (B * N)
 
 
@@ -92,11 +92,12 @@ def time_lif_launches(bb, n=6):
     fn = _lib(bb.device).launch_batch_lif
     times = []
     # launches at offsets 0, n_delay, ... of the next frame; the state just advances (the bench is over)
+    K = bb.steps_per_launch
     for i in range(n):
-        off = (i % max(1, bb.frame_steps // bb.n_delay)) * bb.n_delay
+        off = (i % max(1, bb.frame_steps // K)) * K
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        err = fn(arr, bb._dims, off, bb.n_delay, stream.cuda_stream)
+        err = fn(arr, bb._dims, off, K, stream.cuda_stream)
         e1.record(stream)
         if err:
             raise RuntimeError(f"launch_batch_lif: {err}")
@@ -140,6 +141,17 @@ def optic_training(steps=3, warmup=1):
             "ol_checkpoint": prob.checkpoint, "ol_loss": float(L), "ol_grad_norm": gn}
 
 
+def copy_calibration(dev):
+    """One device-to-device copy of 1 GiB: the kernel of known traffic the PMC byte counters are scaled by."""
+    import torch
+    x = torch.ones(2**28, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log("copy calibration: 1 GiB read + 1 GiB written")
+
+
 def cpu_baseline(conn, retina, lp, op, rollout, frames):
     """The CPU oracle (plain C, one thread) running one fly of the workload for `frames` frames."""
     sys.path.insert(0, str(ROOT))
@@ -163,6 +175,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-optic-training", action="store_true")
+    ap.add_argument("--only-b64", action="store_true",
+                    help="only the B=64 engine (plus --copy-calibration): the short run the PMC counter passes profile")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -207,7 +221,7 @@ def main():
     log(f"B=64: {args.steps} frames, {sim_steps} fly-steps in {gpu_ms:.1f} ms (host wall {wall*1e3:.1f} ms): "
         f"{value:.3f} sim-s/wall-s; rollouts completed {bb.completed}")
 
-    K = bb.n_delay
+    K = bb.steps_per_launch
     lif_ms = time_lif_launches(bb)
     alg_bytes, activity = lif_block_bytes(bb, K, 64)
     achieved = alg_bytes / (lif_ms / 1e3) / 1e9
@@ -215,6 +229,11 @@ def main():
         f"{alg_bytes/1e6:.1f} MB algorithmic -> {achieved:.0f} GB/s")
     del bb
     torch.cuda.empty_cache()
+    if args.only_b64:
+        if args.copy_calibration:
+            copy_calibration(dev)
+        print(json.dumps({"b64_sim_s_per_wall_s": round(value, 4), "lif_launch_ms": round(lif_ms, 4)}), flush=True)
+        return
 
     # ---------------------------------------------------------------- B = 1, plain kernels
     log("B=1 plain path: setup")
@@ -245,11 +264,7 @@ def main():
         torch.cuda.empty_cache()
 
     if args.copy_calibration:
-        x = torch.empty(2**28, dtype=torch.float32, device=dev)
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
-        log("copy calibration: 1 GiB read + 1 GiB written")
+        copy_calibration(dev)
 
     # ---------------------------------------------------------------- CPU oracle on the host cores
     cpu = None

@@ -19,11 +19,6 @@ namespace {
 
 constexpr int NL = 32;   // movies per row: one half-wavefront
 
-__device__ __forceinline__ int half_row(const int* sched, int rows, int pair) {
-    const int k = 2 * pair + (threadIdx.x & 63) / NL;
-    return k < rows ? sched[k] : -1;
-}
-
 // sum over the 32 lanes of a half-wave in a fixed butterfly order (the same value on every lane of the half)
 __device__ __forceinline__ float half_sum(float x) {
     x = x + __shfl_xor(x, 16, 32);
@@ -34,28 +29,59 @@ __device__ __forceinline__ float half_sum(float x) {
     return x;
 }
 
-// chunked fold of row entries [e0, e1) of w * x[col, lane]
-__device__ __forceinline__ float fold32(const int* __restrict__ idx, const float* __restrict__ w,
-                                        const float* __restrict__ x, int e0, int e1, int l) {
+typedef const __attribute__((address_space(4))) int* ci32p;
+typedef const __attribute__((address_space(4))) float* cf32p;
+
+// Chunked fold for the two rows of a wavefront: half 0 folds entries [a0, a1), half 1 entries [b0, b1), each
+// chunk of 16 in order from +0, chunk sums in order from +0.  Both rows' indices and weights are read with scalar
+// loads (the CSR is constant during a launch) and each half selects its own, so an entry costs one vector gather
+// of 32 contiguous floats per half.
+__device__ __forceinline__ float fold32_pair(ci32p idx, cf32p w, const float* __restrict__ x, int a0, int a1, int b0,
+                                             int b1, int l, bool h) {
+    const int n0 = a1 - a0, n1 = b1 - b0, nmax = max(n0, n1);
+    const int nh = h ? n1 : n0;
     float total = 0.0f;
-    for (int cs = e0; cs < e1; cs += 16) {
-        const int n = min(16, e1 - cs);
-        int c[16];
-        float wv[16], xv[16];
+    for (int cs = 0; cs < nmax; cs += 16) {
+        const int m0 = n0 - cs, m1 = n1 - cs;
+        int c0[16], c1[16];
+        float w0[16], w1[16];
+        if (m0 >= 16) {
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            if (j < n) { c[j] = idx[cs + j]; wv[j] = w[cs + j]; }
+            for (int j = 0; j < 16; ++j) { c0[j] = idx[a0 + cs + j]; w0[j] = w[a0 + cs + j]; }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                c0[j] = j < m0 ? idx[a0 + cs + j] : 0;
+                w0[j] = j < m0 ? w[a0 + cs + j] : 0.0f;
+            }
         }
+        if (m1 >= 16) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) { c1[j] = idx[b0 + cs + j]; w1[j] = w[b0 + cs + j]; }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                c1[j] = j < m1 ? idx[b0 + cs + j] : 0;
+                w1[j] = j < m1 ? w[b0 + cs + j] : 0.0f;
+            }
+        }
+        const int m = h ? m1 : m0;
+        float xv[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            if (j < n) xv[j] = x[(size_t)c[j] * NL + l];
+            const int c = h ? c1[j] : c0[j];
+            xv[j] = j < m ? x[(size_t)c * NL + l] : 0.0f;
         }
         float s = 0.0f;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            if (j < n) { float p = wv[j] * xv[j]; s = s + p; }
+            if (j < m) {
+                const float wv = h ? w1[j] : w0[j];
+                float p = wv * xv[j];
+                s = s + p;
+            }
         }
-        total = total + s;
+        if (cs < nh) total = total + s;
     }
     return total;
 }
@@ -102,10 +128,14 @@ struct FwdArgs {
 __global__ __launch_bounds__(256) void k_adj_forward(FwdArgs f) {
     const int l = threadIdx.x & (NL - 1);
     const int pairs = (f.rows + 1) / 2;
-    for (int pr = blockIdx.x * 4 + (threadIdx.x >> 6); pr < pairs; pr += gridDim.x * 4) {
-        const int r = half_row(f.sched, f.rows, pr);
+    const bool h = (threadIdx.x & 63) >= NL;
+    for (int pr = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); pr < pairs; pr += gridDim.x * 4) {
+        const int r0 = f.sched[2 * pr];
+        const int r1 = 2 * pr + 1 < f.rows ? f.sched[2 * pr + 1] : -1;
+        const int r = h ? r1 : r0;
+        const float y = fold32_pair((ci32p)f.idx, (cf32p)f.w, f.dr, f.ptr[r0], f.ptr[r0 + 1], r1 < 0 ? 0 : f.ptr[r1],
+                                    r1 < 0 ? 0 : f.ptr[r1 + 1], l, h);
         if (r < 0) continue;
-        const float y = fold32(f.idx, f.w, f.dr, f.ptr[r], f.ptr[r + 1], l);
         const int64_t q = (int64_t)r * NL + l;
         float inp = f.g_rr * y;
         inp = inp + f.P[q];
@@ -176,10 +206,14 @@ struct BackArgs {
 __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
     const int l = threadIdx.x & (NL - 1);
     const int pairs = (k.rows + 1) / 2;
-    for (int pr = blockIdx.x * 4 + (threadIdx.x >> 6); pr < pairs; pr += gridDim.x * 4) {
-        const int j = half_row(k.schedT, k.rows, pr);
+    const bool h = (threadIdx.x & 63) >= NL;
+    for (int pr = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); pr < pairs; pr += gridDim.x * 4) {
+        const int j0 = k.schedT[2 * pr];
+        const int j1 = 2 * pr + 1 < k.rows ? k.schedT[2 * pr + 1] : -1;
+        const int j = h ? j1 : j0;
+        const float s = fold32_pair((ci32p)k.idxT, (cf32p)k.wT, k.ly, k.ptrT[j0], k.ptrT[j0 + 1],
+                                    j1 < 0 ? 0 : k.ptrT[j1], j1 < 0 ? 0 : k.ptrT[j1 + 1], l, h);
         if (j < 0) continue;
-        const float s = fold32(k.idxT, k.wT, k.ly, k.ptrT[j], k.ptrT[j + 1], l);
         const int64_t q = (int64_t)j * NL + l;
         const float la1 = k.la[q], lv1 = k.lv[q];
         float c = la1 * (1.0f - k.a_ad);
@@ -263,14 +297,21 @@ __global__ __launch_bounds__(256) void k_segment_sum(const float* __restrict__ v
     if (lane == 0) out[k] = total;
 }
 
-// loss = sum over all (step, readout) partial sums, in order, times scale.
-__global__ void k_loss_total(const float* __restrict__ parts, int64_t n, float scale, float* __restrict__ out) {
-    const int lane = threadIdx.x;
+// loss = scale * sum of the (step, readout) partial sums: 1024 threads each fold a strided slice in order, then
+// the 1024 slice sums are folded in thread order (a fixed order for a given n).
+__global__ __launch_bounds__(1024) void k_loss_total(const float* __restrict__ parts, int64_t n, float scale,
+                                                     float* __restrict__ out) {
+    __shared__ float slice[1024];
+    const int tid = threadIdx.x;
     float s = 0.0f;
-    for (int64_t m = lane; m < n; m += 64) s = s + parts[m];
-    float total = 0.0f;
-    for (int j = 0; j < 64; ++j) total = total + fh_readlane(s, j);
-    if (lane == 0) out[0] = total * scale;
+    for (int64_t m = tid; m < n; m += 1024) s = s + parts[m];
+    slice[tid] = s;
+    __syncthreads();
+    if (tid == 0) {
+        float total = 0.0f;
+        for (int j = 0; j < 1024; ++j) total = total + slice[j];
+        out[0] = total * scale;
+    }
 }
 
 inline unsigned nblk(int64_t total, int per = 256) { return (unsigned)((total + per - 1) / per); }
@@ -367,7 +408,7 @@ FH_EXPORT int launch_segment_sum(void** p, int nseg, void* stream) {
 // p = [parts, out]
 FH_EXPORT int launch_loss_total(void** p, int64_t n, float scale, void* stream) {
     if (n < 0) return FH_ERR_ARGS;
-    hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(64), 0, (hipStream_t)stream, (const float*)p[0], n, scale,
+    hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(1024), 0, (hipStream_t)stream, (const float*)p[0], n, scale,
                        (float*)p[1]);
     FH_LAUNCH_CHECK();
 }

@@ -2,7 +2,9 @@
 //
 // State is (rows, 64): row r, lane b holds fly b's value of cell r, so a wavefront reads and writes one row of all
 // flies with one coalesced 256-byte access.  A spike is one bit: after a LIF step the wavefront's ballot of "fired"
-// is the 64-bit mask of which flies' copy of this cell spiked, stored in a ring of masks (ring_slots, rows).
+// is the 64-bit mask of which flies' copy of this cell spiked, stored in a ring of masks.  Two layouts: slot-major
+// (ring_slots, rows), where all wavefronts reading the same step share one small slot, and row-major
+// (rows, ring_slots), where a cache line holds one cell's masks for consecutive steps; the autotuner picks.
 //
 // The synaptic delay (n_delay steps, 18 at dt 0.1 ms) makes every row independent for n_delay consecutive steps:
 // the input of step t comes from spikes of step t - n_delay, which a previous launch wrote.  So one launch runs up
@@ -162,7 +164,7 @@ __device__ __forceinline__ float fold_edges_transposed(float acc, const uint32_t
 
 struct BatchLif {
     const int* ptr;          // (n_rows + 1) post-major CSR over LIF rows
-    const uint32_t* pre_off; // (nnz) byte offset of the presynaptic row's mask within a ring slot (row * 8)
+    const uint32_t* pre_off; // (nnz) byte offset of the presynaptic row's masks in the ring (row * ring_slots * 8)
     const float* w;          // (nnz) weights (mV)
     const int* sched;        // (n_rows) visiting order
     const int* neuron_id;    // (n_rows) global neuron index (Philox counter)
@@ -175,7 +177,7 @@ struct BatchLif {
     const uint32_t* budget;  // (64) steps this lane's fly may take in total
     const uint32_t* engine_step; // (1) engine step at the start of the frame (ring slot clock)
     const float* P;          // (12) v_rest, v_reset, v_th, a_m, dt, t_ref, -, -, -, a_r, rate_gain, a_s
-    int n_rows, lanes, ring_slots, delay, record, mode;
+    int n_rows, lanes, ring_slots, delay, record, mode, row_major;
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -210,7 +212,8 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
         const float p = has_pois ? a.pois[q] : 0.0f;
         for (int k = 0; k < K; ++k) {
             const uint32_t t = T0 + (uint32_t)k;
-            const char* rb = (const char*)(a.ring + (size_t)((t + R - D) % R) * a.n_rows);
+            const uint32_t rslot = (t + R - D) % R;
+            const char* rb = (const char*)(a.row_major ? a.ring + rslot : a.ring + (size_t)rslot * a.n_rows);
             float acc = g * a_s;
             if (a.mode == 2)
                 acc = fold_edges_transposed(acc, a.pre_off, a.w, rb, e0, e1, lane, wstage[threadIdx.x >> 6]);
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                 cnt = cnt + spike;
             }
             const uint64_t bits = __builtin_amdgcn_ballot_w64(live && fired);
-            if (lane == 0) a.ring[(size_t)(t % R) * a.n_rows + r] = bits;
+            if (lane == 0) a.ring[a.row_major ? (size_t)r * R + t % R : (size_t)(t % R) * a.n_rows + r] = bits;
         }
         a.v[q] = v; a.g[q] = g; a.refrac[q] = ref; a.rate[q] = rate;
         if (a.record) a.counts[q] = cnt;
@@ -455,7 +458,7 @@ inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
 
 // Up to `delay` LIF steps of every row for 64 flies.  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v, g,
 // refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12)];
-// dims = [n_rows, lanes, ring_slots, delay, record, grid, mode]; mode 0 reads the spike masks through the scalar
+// dims = [n_rows, lanes, ring_slots, delay, record, grid, mode, row_major]; mode 0 reads the spike masks through the scalar
 // cache, mode 1 gathers them with vector loads, mode 2 gathers and transposes them (fold_edges,
 // fold_edges_gather, fold_edges_transposed: the same additions in the same order, so the same result).
 FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream) {
@@ -469,6 +472,7 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     a.n_rows = dims[0]; a.lanes = dims[1]; a.ring_slots = dims[2]; a.delay = dims[3]; a.record = dims[4];
     const int grid = dims[5];
     a.mode = dims[6];
+    a.row_major = dims[7];
     if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
         grid <= 0)
         return FH_ERR_ARGS;

@@ -1,0 +1,74 @@
+"""Launch-shape autotuner for the batched engine and the packed-subgraph stepper.
+
+Every shape it chooses between performs the same additions in the same order (the fold modes, the steps per
+launch, the grid size, the rounds per pull), so results are bit-identical whichever wins; only speed differs.  The
+winner is cached in a JSON file keyed by the graph's shape, so a rerun on the same workload skips the search.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+from pathlib import Path
+
+import torch
+
+CACHE = Path(os.environ.get("FLYHIP_TUNE_CACHE", Path.home() / ".cache" / "flyhip" / "tune.json"))
+
+
+def _load() -> dict:
+    try:
+        return json.loads(CACHE.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(table: dict) -> None:
+    try:
+        CACHE.parent.mkdir(parents=True, exist_ok=True)
+        tmp = CACHE.with_suffix(f".tmp{os.getpid()}")
+        tmp.write_text(json.dumps(table, indent=1, sort_keys=True))
+        tmp.replace(CACHE)
+    except OSError:
+        pass
+
+
+def batch_key(bb) -> str:
+    dev = torch.cuda.get_device_properties(bb.device)
+    return f"batch:{dev.name}:{dev.multi_processor_count}:rows{bb.n_rows}:nnz{bb.W_lif.nnz}:rate{bb.n_rate}:D{bb.n_delay}"
+
+
+def tune_batch(bb, frames: int = 3, fold_modes=(1, 2), steps=(3, 4, 6, 9), grids=(16, 32), use_cache=True,
+               log=None) -> dict:
+    """Time each (fold mode, steps per launch, grid per CU) on `bb` (a BatchBrain with rollouts queued) and leave
+    it configured with the fastest.  Returns {"config": ..., "ms_per_frame": ..., "trials": [...]}."""
+    key = batch_key(bb)
+    table = _load() if use_cache else {}
+    if key in table:
+        cfg = table[key]["config"]
+        bb.configure(**cfg)
+        return table[key] | {"cached": True}
+    trials = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for mode, K, gpc in itertools.product(fold_modes, steps, grids):
+        if K > bb.n_delay:
+            continue
+        bb.configure(fold_mode=mode, steps_per_launch=K, grid_per_cu=gpc)
+        bb.run(1)                                   # capture + first replay
+        torch.cuda.synchronize()
+        e0.record()
+        bb.run(frames)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / frames
+        trials.append({"fold_mode": mode, "steps_per_launch": K, "grid_per_cu": gpc, "ms_per_frame": round(ms, 4)})
+        if log:
+            log(f"  tune: fold {mode} K {K} grid x{gpc}: {ms:.3f} ms/frame")
+    best = min(trials, key=lambda t: t["ms_per_frame"])
+    cfg = {k: best[k] for k in ("fold_mode", "steps_per_launch", "grid_per_cu")}
+    bb.configure(**cfg)
+    result = {"config": cfg, "ms_per_frame": best["ms_per_frame"], "trials": trials}
+    if use_cache:
+        table[key] = result
+        _save(table)
+    return result
