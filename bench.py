@@ -11,6 +11,8 @@ Shiu LIF at dt 0.1 ms coupled to the 89,390-unit optic lobe every 10 ms frame (1
   * roofline: the batched LIF launch (csrc/batch.hip k_batch_lif, <= 18 steps per launch), timed with HIP events
     on its stream after the timed region; algorithmic bytes per launch from the per-step LIF and synaptic-input
     byte counts.
+  * packed: 128 LIF subgraphs (5k-80k cells, log-uniform; in-degree of the full graph) packed into one CSR and
+    stepped 5,000 steps by one cooperative launch of csrc/packed.hip, launch shape from flyhip.autotune.
   * cpu_baseline: the CPU oracle (oracle/, plain C, one thread) on one fly of the same workload for a bounded sample
     of frames, scaled to sim-s/wall-s.
 """
@@ -141,6 +143,45 @@ def optic_training(steps=3, warmup=1):
             "ol_checkpoint": prob.checkpoint, "ol_loss": float(L), "ol_grad_norm": gn}
 
 
+def packed_run(steps, log):
+    """128 uneven subgraphs, each with its own seed and 300 Poisson inputs at U[0, 200] Hz, `steps` LIF steps in one
+    launch after a tuning pass."""
+    import torch
+    from flyhip import synth
+    from flyhip.autotune import tune_packed
+    from flyhip.brain import LIFParams
+    from flyhip.packed import PackedBrains
+    t0 = time.perf_counter()
+    mats = synth.packed_subgraphs(3, 128, 5_000, 80_000, mean_in_degree=153.0)
+    rng = np.random.default_rng(12)
+    inputs = [(np.sort(rng.choice(m.shape[0], 300, replace=False)), rng.uniform(0, 200, 300).astype(np.float32))
+              for m in mats]
+    seeds = rng.integers(1, 2**62, len(mats), dtype=np.uint64)
+    lp = LIFParams.shiu(dt=0.1)
+    pb = PackedBrains(mats, lp, seeds=seeds, inputs=inputs, device="cuda")
+    del mats
+    log(f"packed: {pb.k} subgraphs, {pb.n} cells, {pb.nnz} edges built in {time.perf_counter() - t0:.1f} s")
+    tune = tune_packed(pb, use_cache=False, log=log)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    pb.run(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    pb.check()
+    ms = e0.elapsed_time(e1)
+    sim = pb.k * steps * lp.dt / 1000.0 / (ms / 1000.0)
+    rate = float(pb.counts.sum().item()) / pb.n / (steps * lp.dt / 1000.0)
+    log(f"packed: {steps} steps in one launch, {ms:.1f} ms ({ms * 1000 / steps:.1f} us/step): "
+        f"{sim:.2f} instance-sim-s/wall-s, mean rate {rate:.1f} Hz")
+    out = {"packed_subgraphs": pb.k, "packed_cells": pb.n, "packed_edges": pb.nnz, "packed_steps": steps,
+           "packed_ms": round(ms, 2), "packed_us_per_step": round(ms * 1000 / steps, 2),
+           "packed_instance_sim_s_per_wall_s": round(sim, 3), "packed_window": pb.window, "packed_grid": pb.grid,
+           "packed_mean_rate_hz": round(rate, 2)}
+    del pb
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_calibration(dev):
     """One device-to-device copy of 1 GiB: the kernel of known traffic the PMC byte counters are scaled by."""
     import torch
@@ -172,7 +213,10 @@ def main():
     ap.add_argument("--steps", type=int, default=60, help="timed 10 ms frames of the B=64 engine")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--b1-frames", type=int, default=30)
-    ap.add_argument("--cpu-frames", type=int, default=12)
+    ap.add_argument("--cpu-frames", type=int, default=60)
+    ap.add_argument("--packed-steps", type=int, default=5000)
+    ap.add_argument("--no-packed", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape of the B=64 engine")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-optic-training", action="store_true")
     ap.add_argument("--only-b64", action="store_true",
@@ -205,6 +249,11 @@ def main():
     bb = BatchBrain(conn, retina, lp, op, device=dev)
     bb.submit(rollouts)
     bb.prepare(graph=True)
+    tuned = None
+    if not args.no_tune:
+        from flyhip.autotune import tune_batch
+        tuned = tune_batch(bb, use_cache=False, log=log)["config"]
+        log(f"B=64 launch shape: {tuned}")
     bb.run(args.warmup)
     torch.cuda.synchronize()
     s0 = bb.simulated_steps
@@ -263,6 +312,11 @@ def main():
             f"({ol['ol_us_per_simulated_step']} us per simulated step)")
         torch.cuda.empty_cache()
 
+    packed = None
+    if not args.no_packed:
+        log("packed uneven subgraphs ...")
+        packed = packed_run(args.packed_steps, log)
+
     if args.copy_calibration:
         copy_calibration(dev)
 
@@ -289,7 +343,7 @@ def main():
                    "batch": 64, "neurons": int(conn.n), "edges": int(conn.W.nnz), "dt_ms": lp.dt,
                    "b1_sim_s_per_wall_s": round(b1, 4), "ratio_b64_vs_b1": round(value / b1, 2),
                    "b1_ms_per_frame": round(b1_ms / args.b1_frames, 4), "spike_activity_per_step": activity,
-                   **(ol or {})},
+                   "b64_launch_shape": tuned, **(ol or {}), **(packed or {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_batch_lif", "launch_ms": round(lif_ms, 4), "steps_per_launch": K,

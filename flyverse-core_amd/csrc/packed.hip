@@ -1,6 +1,6 @@
-// Packed uneven subgraphs in one persistent launch: many independent LIF graphs (one fly each, sizes uneven) are
-// packed block-diagonally into one CSR with offset tables and stepped for a whole rollout by a single cooperative
-// launch.  An addition: the reference runs every Connectome.subset as its own Brain (flyverse/connectome.py:469-481).
+// Packed uneven subgraphs: many independent LIF graphs (one fly each, sizes uneven) are packed block-diagonally into
+// one CSR with offset tables and stepped together.  An addition: the reference runs every Connectome.subset as its
+// own Brain (flyverse/connectome.py:469-481).
 //
 // Per step and target row, the synaptic input keeps the reference CPU order (flyverse/brain.py:832-833, 679-692):
 // g = g*a_s, then the in-edges whose presynaptic cell fired n_delay steps ago, in ascending presynaptic order; the
@@ -13,6 +13,10 @@
 //
 // The Poisson uniform of a cell is Philox keyed by its instance's seed with counter (local cell index, step, 0, 0):
 // every instance draws exactly what it would draw run alone as a one-fly brain.
+//
+// Two drivers share the window body: launch_packed_window runs one window per launch (the kernel boundary is the
+// synchronisation; a rollout is a captured graph of such launches with the step base on the device), and
+// launch_packed_run runs a whole rollout in one cooperative launch with a grid barrier between windows.
 #include "common.h"
 
 namespace {
@@ -33,16 +37,17 @@ struct Packed {
     uint64_t* ring;          // (kSlots, words) fired bits
     unsigned* bar_count;     // grid barrier arrivals
     unsigned* bar_gen;       // grid barrier generation
-    int* error;              // set when a barrier wait exceeds its bound
+    int* error;              // (4) set when a barrier wait exceeds its bound: flag, arrivals seen, grid, window
     const float* P;          // (12) v_rest, v_reset, v_th, a_m, dt, t_ref, -, -, -, a_r, rate_gain, a_s
     int n, words, steps, window, delay, t0;
+    uint64_t spin_ticks;     // bound of one barrier wait, in wall-clock ticks
 };
 
 // Grid-wide barrier over a cooperative launch.  Release fence before arriving (this window's ring bits and state are
 // visible device-wide), acquire fence after leaving (no stale vector-L1 lines of the ring survive into the next
-// window).  The wait is bounded; once any wait has timed out, every later barrier returns at once, so a grid that
-// is not fully resident ends the kernel in well under a second with the error flag set instead of hanging.
-__device__ __forceinline__ void grid_barrier(const Packed& a, unsigned nblocks) {
+// window).  The wait is bounded in wall-clock time; once any wait has timed out, every later barrier returns at once,
+// so a grid that is not fully resident ends the kernel with the error flag set instead of hanging.
+__device__ __forceinline__ void grid_barrier(const Packed& a, unsigned nblocks, int T0) {
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -52,10 +57,17 @@ __device__ __forceinline__ void grid_barrier(const Packed& a, unsigned nblocks) 
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             atomicAdd(a.bar_gen, 1u);
         } else if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-            int spins = 0;
+            const uint64_t start = wall_clock64();
             while (__hip_atomic_load(a.bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
-                if (++spins > (1 << 22)) { atomicExch(a.error, 1); break; }
-                __builtin_amdgcn_s_sleep(1);
+                if (wall_clock64() - start > a.spin_ticks) {
+                    if (atomicExch(a.error, 1) == 0) {
+                        a.error[1] = (int)__hip_atomic_load(a.bar_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        a.error[2] = (int)nblocks;
+                        a.error[3] = T0;
+                    }
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
             }
         }
     }
@@ -63,7 +75,8 @@ __device__ __forceinline__ void grid_barrier(const Packed& a, unsigned nblocks) 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-__global__ __launch_bounds__(256) void k_packed_run(Packed a) {
+// One window: steps T0 .. T0 + K - 1 of every row.  Also clears the ring slots of the window that follows it.
+__device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = gridDim.x * 4;
@@ -71,113 +84,164 @@ __global__ __launch_bounds__(256) void k_packed_run(Packed a) {
     const int nthreads = gridDim.x * blockDim.x;
     const float v_rest = a.P[0], v_reset = a.P[1], v_th = a.P[2], am = a.P[3], dt = a.P[4], t_ref = a.P[5];
     const float ar = a.P[9], rg = a.P[10], a_s = a.P[11];
-    for (int T0 = a.t0; T0 < a.t0 + a.steps; T0 += a.window) {
-        const int K = min(a.window, a.t0 + a.steps - T0);
-        // clear the slots the next window writes (steps T0 + K .. T0 + K + window - 1; after the last window of a
-        // launch these are the first window of a launch that continues it with the same window)
-        for (int64_t q = gthread; q < (int64_t)a.window * a.words; q += nthreads) {
-            const int s = (T0 + K + (int)(q / a.words)) & (kSlots - 1);
-            a.ring[(int64_t)s * a.words + q % a.words] = 0ull;
-        }
-        for (int si = wave; si < a.n; si += nwaves) {
-            const int r = a.sched[si];
-            const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
-            float v = a.v[r], g = a.g[r], ref = a.refrac[r], rate = a.rate[r], cnt = a.counts[r];
-            const float drive = a.drive[r], p = a.pois[r];
-            const uint64_t seed = a.seeds[a.inst[r]];
-            const uint32_t loc = (uint32_t)a.local[r];
-            for (int k = 0; k < K; ++k) {
-                const int t = T0 + k;
-                const uint64_t* slot = a.ring + (int64_t)((t - a.delay) & (kSlots - 1)) * a.words;
-                float acc = g * a_s;
-                for (int base = e0; base < e1; base += 4 * kWave) {
-                    int pre[4];
-                    bool act[4];
+    // clear the slots the next window writes (steps T0 + K .. T0 + K + window - 1)
+    for (int64_t q = gthread; q < (int64_t)a.window * a.words; q += nthreads) {
+        const int s = (T0 + K + (int)(q / a.words)) & (kSlots - 1);
+        a.ring[(int64_t)s * a.words + q % a.words] = 0ull;
+    }
+    for (int si = wave; si < a.n; si += nwaves) {
+        const int r = a.sched[si];
+        const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
+        float v = a.v[r], g = a.g[r], ref = a.refrac[r], rate = a.rate[r], cnt = a.counts[r];
+        const float drive = a.drive[r], p = a.pois[r];
+        const uint64_t seed = a.seeds[a.inst[r]];
+        const uint32_t loc = (uint32_t)a.local[r];
+        for (int k = 0; k < K; ++k) {
+            const int t = T0 + k;
+            const uint64_t* slot = a.ring + (int64_t)((t - a.delay) & (kSlots - 1)) * a.words;
+            float acc = g * a_s;
+            for (int base = e0; base < e1; base += 4 * kWave) {
+                int pre[4];
+                bool act[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int e = base + u * kWave + lane;
-                        pre[u] = e < e1 ? a.idx[e] : -1;
-                    }
+                for (int u = 0; u < 4; ++u) {
+                    const int e = base + u * kWave + lane;
+                    pre[u] = e < e1 ? a.idx[e] : -1;
+                }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        act[u] = pre[u] >= 0 && ((slot[pre[u] >> 6] >> (pre[u] & 63)) & 1ull);
+                for (int u = 0; u < 4; ++u)
+                    act[u] = pre[u] >= 0 && ((slot[pre[u] >> 6] >> (pre[u] & 63)) & 1ull);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (base + u * kWave >= e1) break;
-                        const float c = act[u] ? a.w[base + u * kWave + lane] : 0.0f;
-                        uint64_t m = __builtin_amdgcn_ballot_w64(act[u]);
-                        while (m) {
-                            const int j = __builtin_ctzll(m);
-                            acc = acc + fh_readlane(c, j);
-                            m &= m - 1;
-                        }
+                for (int u = 0; u < 4; ++u) {
+                    if (base + u * kWave >= e1) break;
+                    const float c = act[u] ? a.w[base + u * kWave + lane] : 0.0f;
+                    uint64_t m = __builtin_amdgcn_ballot_w64(act[u]);
+                    while (m) {
+                        const int j = __builtin_ctzll(m);
+                        acc = acc + fh_readlane(c, j);
+                        m &= m - 1;
                     }
                 }
-                // LIF update (uniform across the wavefront: every lane computes the same row)
-                float target = v_rest + acc;
-                target = target + drive;
-                float dv = v - target;
-                dv = dv * am;
-                float vn = target + dv;
-                float rn = ref;
-                if (rn > 0.0f) vn = v_reset;
-                rn = rn - dt;
-                if (rn < 0.0f) rn = 0.0f;
-                float spike = (vn >= v_th) ? 1.0f : 0.0f;
-                if (p > 0.0f && fh_uniform(seed, (uint32_t)t, loc, 0u, 0u) < p) spike = 1.0f;
-                const bool fired = spike > 0.0f;
-                if (fired) { vn = v_reset; rn = t_ref; }
-                float rr = rate * ar;
-                float rs = spike * rg;
-                g = acc; v = vn; ref = rn; rate = rr + rs; cnt = cnt + spike;
-                if (fired && lane == 0)
-                    atomicOr((unsigned long long*)(a.ring + (int64_t)(t & (kSlots - 1)) * a.words + (r >> 6)),
-                             1ull << (r & 63));
             }
-            if (lane == 0) { a.v[r] = v; a.g[r] = g; a.refrac[r] = ref; a.rate[r] = rate; a.counts[r] = cnt; }
+            // LIF update (uniform across the wavefront: every lane computes the same row)
+            float target = v_rest + acc;
+            target = target + drive;
+            float dv = v - target;
+            dv = dv * am;
+            float vn = target + dv;
+            float rn = ref;
+            if (rn > 0.0f) vn = v_reset;
+            rn = rn - dt;
+            if (rn < 0.0f) rn = 0.0f;
+            float spike = (vn >= v_th) ? 1.0f : 0.0f;
+            if (p > 0.0f && fh_uniform(seed, (uint32_t)t, loc, 0u, 0u) < p) spike = 1.0f;
+            const bool fired = spike > 0.0f;
+            if (fired) { vn = v_reset; rn = t_ref; }
+            float rr = rate * ar;
+            float rs = spike * rg;
+            g = acc; v = vn; ref = rn; rate = rr + rs; cnt = cnt + spike;
+            if (fired && lane == 0)
+                atomicOr((unsigned long long*)(a.ring + (int64_t)(t & (kSlots - 1)) * a.words + (r >> 6)),
+                         1ull << (r & 63));
         }
-        grid_barrier(a, gridDim.x);
+        if (lane == 0) { a.v[r] = v; a.g[r] = g; a.refrac[r] = ref; a.rate[r] = rate; a.counts[r] = cnt; }
     }
 }
 
-}  // namespace
+__global__ __launch_bounds__(256) void k_packed_run(Packed a) {
+    for (int T0 = a.t0; T0 < a.t0 + a.steps; T0 += a.window) {
+        packed_window(a, T0, min(a.window, a.t0 + a.steps - T0));
+        grid_barrier(a, gridDim.x, T0);
+    }
+}
 
-// Run `steps` LIF steps (from global step t0) of the packed graph in one cooperative launch.
-// p = [ptr, idx, w, sched, v, g, refrac, rate, counts, drive, pois, inst, local, seeds, ring (64, words),
-//      bar (2 uint32: count, generation), error (int32), P (12)];
-// dims = [n, steps, window, delay, grid, t0].  Requires window <= delay and 2 * delay + window <= 64; a run that
-// continues an earlier one (t0 > 0) must use the same window, whose last window cleared the next slots.
-FH_EXPORT int launch_packed_run(void** p, const int* dims, void* stream) {
-    Packed a;
+// One window per launch; the window's first step is *t_base + off.
+__global__ __launch_bounds__(256) void k_packed_window(Packed a, const int* t_base, int off, int K) {
+    packed_window(a, *t_base + off, K);
+}
+
+__global__ void k_advance(int* t_base, int steps) {
+    if (threadIdx.x == 0) *t_base += steps;
+}
+
+static void fill_packed(Packed& a, void** p) {
     a.ptr = (const int*)p[0]; a.idx = (const int*)p[1]; a.w = (const float*)p[2]; a.sched = (const int*)p[3];
     a.v = (float*)p[4]; a.g = (float*)p[5]; a.refrac = (float*)p[6]; a.rate = (float*)p[7]; a.counts = (float*)p[8];
     a.drive = (const float*)p[9]; a.pois = (const float*)p[10]; a.inst = (const int*)p[11];
     a.local = (const int*)p[12]; a.seeds = (const uint64_t*)p[13]; a.ring = (uint64_t*)p[14];
     a.bar_count = (unsigned*)p[15]; a.bar_gen = (unsigned*)p[15] + 1; a.error = (int*)p[16];
     a.P = (const float*)p[17];
+}
+
+// Workgroups of k_packed_run that are resident together: one per CU fewer than the occupancy query reports (on
+// MI355X it reports 7 per CU for this kernel, yet at most 6 per CU were ever resident at once; the bounded barrier
+// wait reports such a shortfall as an error instead of hanging).
+static int resident_grid(int per_cu, int cus) { return (per_cu > 1 ? per_cu - 1 : per_cu) * cus; }
+
+static bool packed_shape_ok(const Packed& a) {
+    return a.n >= 0 && a.steps >= 0 && a.window >= 1 && a.window <= a.delay && 2 * a.delay + a.window <= kSlots &&
+           2 * a.window + a.delay <= kSlots && a.t0 >= 0;
+}
+
+}  // namespace
+
+// Run `steps` LIF steps (from global step t0) of the packed graph in one cooperative launch.
+// p = [ptr, idx, w, sched, v, g, refrac, rate, counts, drive, pois, inst, local, seeds, ring (64, words),
+//      bar (2 uint32: count, generation), error (4 int32), P (12)];
+// dims = [n, steps, window, delay, grid, t0, barrier bound in ms].  Requires window <= delay and
+// 2 * delay + window <= 64; a run that continues an earlier one (t0 > 0) must use the same window, whose last window
+// cleared the next slots.
+FH_EXPORT int launch_packed_run(void** p, const int* dims, void* stream) {
+    Packed a;
+    fill_packed(a, p);
     a.n = dims[0]; a.steps = dims[1]; a.window = dims[2]; a.delay = dims[3];
     const int grid = dims[4];
     a.t0 = dims[5];
     a.words = (a.n + 63) / 64;
-    if (a.n < 0 || a.steps < 0 || a.window < 1 || a.window > a.delay || 2 * a.delay + a.window > kSlots ||
-        2 * a.window + a.delay > kSlots || grid <= 0 || a.t0 < 0)
-        return FH_ERR_ARGS;
+    if (!packed_shape_ok(a) || grid <= 0 || dims[6] <= 0) return FH_ERR_ARGS;
     if (a.n == 0 || a.steps == 0) return 0;
-    int per_cu = 0, cus = 0, dev = 0;
+    int per_cu = 0, cus = 0, dev = 0, khz = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return (int)e;
+    e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    if (e != hipSuccess) return (int)e;
+    a.spin_ticks = (uint64_t)khz * (uint64_t)dims[6];
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_packed_run, 256, 0);
     if (e != hipSuccess) return (int)e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return (int)e;
-    if (grid > per_cu * cus) return FH_ERR_SIZE;          // every block must be resident for the grid barrier
+    if (grid > resident_grid(per_cu, cus)) return FH_ERR_SIZE;   // every block must be resident for the barrier
     void* args[] = {&a};
     e = hipLaunchCooperativeKernel((const void*)k_packed_run, dim3(grid), dim3(256), args, 0, (hipStream_t)stream);
     if (e != hipSuccess) return (int)e;
     FH_LAUNCH_CHECK();
 }
 
-// The most workgroups of k_packed_run that can be resident at once (the largest valid grid).
+// One window of K steps starting at step *t_base + off (t_base: int32 on the device, read when the kernel runs, so a
+// captured graph of windows replays at advancing steps).  p as launch_packed_run (bar and error unused);
+// dims = [n, K, window, delay, grid, off].  The kernel boundary orders the window against the next one.
+FH_EXPORT int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream) {
+    Packed a;
+    fill_packed(a, p);
+    a.n = dims[0]; a.steps = dims[1]; a.window = dims[2]; a.delay = dims[3];
+    const int grid = dims[4], off = dims[5];
+    a.t0 = 0;
+    a.words = (a.n + 63) / 64;
+    a.spin_ticks = 0;
+    if (!packed_shape_ok(a) || a.steps > a.window || grid <= 0 || off < 0 || t_base == nullptr) return FH_ERR_ARGS;
+    if (a.n == 0 || a.steps == 0) return 0;
+    hipLaunchKernelGGL(k_packed_window, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, t_base, off, a.steps);
+    FH_LAUNCH_CHECK();
+}
+
+// *t_base += steps on the stream (the last node of a captured rollout).
+FH_EXPORT int launch_packed_advance(int* t_base, int steps, void* stream) {
+    if (t_base == nullptr) return FH_ERR_ARGS;
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, (hipStream_t)stream, t_base, steps);
+    FH_LAUNCH_CHECK();
+}
+
+// The largest grid launch_packed_run accepts (workgroups resident together).
 FH_EXPORT int packed_max_grid(int* out) {
     int per_cu = 0, cus = 0, dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -186,6 +250,6 @@ FH_EXPORT int packed_max_grid(int* out) {
     if (e != hipSuccess) return (int)e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return (int)e;
-    *out = per_cu * cus;
+    *out = resident_grid(per_cu, cus);
     return 0;
 }

@@ -72,3 +72,45 @@ def tune_batch(bb, frames: int = 3, fold_modes=(1, 2), steps=(3, 4, 6, 9), grids
         table[key] = result
         _save(table)
     return result
+
+
+def packed_key(pb) -> str:
+    dev = torch.cuda.get_device_properties(pb.device)
+    return f"packed:{dev.name}:{dev.multi_processor_count}:n{pb.n}:nnz{pb.nnz}:k{pb.k}:D{pb.n_delay}"
+
+
+def tune_packed(pb, steps: int = 180, windows=None, grids=None, use_cache=True, log=None) -> dict:
+    """Time each (window, grid) of the packed stepper `pb` (in its current mode) over `steps` steps from rest and
+    leave it configured with the fastest (and reset to step 0).  Returns {"config": ..., "ms_per_step": ...,
+    "trials": [...]}."""
+    key = packed_key(pb)
+    table = _load() if use_cache else {}
+    if key in table:
+        pb.configure(**table[key]["config"])
+        return table[key] | {"cached": True}
+    cus = torch.cuda.get_device_properties(pb.device).multi_processor_count
+    windows = windows or sorted({w for w in (pb.n_delay, (pb.n_delay + 1) // 2) if w >= 1})
+    grids = grids or sorted({min(pb.max_grid, cus * m) for m in (2, 4, 6)})
+    trials = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for w, g in itertools.product(windows, grids):
+        pb.configure(window=w, grid=g)
+        pb.run(steps)                               # graph capture + warm caches
+        pb.configure(window=w, grid=g)
+        e0.record()
+        pb.run(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        pb.check()
+        ms = e0.elapsed_time(e1) / steps
+        trials.append({"window": w, "grid": g, "ms_per_step": round(ms, 5)})
+        if log:
+            log(f"  tune: window {w} grid {g}: {ms * 1000:.1f} us/step")
+    best = min(trials, key=lambda t: t["ms_per_step"])
+    cfg = {"window": best["window"], "grid": best["grid"]}
+    pb.configure(**cfg)
+    result = {"config": cfg, "ms_per_step": best["ms_per_step"], "trials": trials}
+    if use_cache:
+        table[key] = result
+        _save(table)
+    return result
