@@ -6,13 +6,14 @@ Shiu LIF at dt 0.1 ms coupled to the 89,390-unit optic lobe every 10 ms frame (1
     photoreceptor movie seed, 300 Poisson inputs at U[0, 200] Hz and a length uniform over 2,000-10,000 LIF steps
     (whole 10 ms frames); finished lanes are refilled.  One bench step = one 10 ms frame of all 64 lanes, replayed
     from a captured HIP graph.  value = simulated fly-seconds / wall-seconds over the timed frames.
-  * B = 1: flyhip.FlyBrain(batch=1, cuda_graphs=True), the plain kernels (launch_pull + launch_lif_rng per step),
+  * B = 1: flyhip.FlyBrain(batch=1, cuda_graphs=True), the plain kernels (launch_pull_bits + launch_lif_bits per step),
     one captured graph per frame shape; reported as b1_sim_s_per_wall_s and the ratio.
-  * roofline: the batched LIF launch (csrc/batch.hip k_batch_lif, <= 18 steps per launch), timed with HIP events
-    on its stream after the timed region; algorithmic bytes per launch from the per-step LIF and synaptic-input
-    byte counts.
+  * roofline: the batched LIF launch (csrc/batch.hip k_batch_lif, <= n_delay steps per launch): one frame of its
+    launches timed with HIP events on the launch stream right after the timed region; algorithmic bytes per launch
+    from the per-step LIF and synaptic-input byte counts (lif_block_bytes).
   * packed: 128 LIF subgraphs (5k-80k cells, log-uniform; in-degree of the full graph) packed into one CSR and
-    stepped 5,000 steps by one cooperative launch of csrc/packed.hip, launch shape from flyhip.autotune.
+    stepped 5,000 steps by one replay of a captured graph of window launches (csrc/packed.hip), window and grid
+    from flyhip.autotune.
   * cpu_baseline: the CPU oracle (oracle/, plain C, one thread) on one fly of the same workload for a bounded sample
     of frames, scaled to sim-s/wall-s.
 """
@@ -63,7 +64,8 @@ def cpu_model():
 
 
 def lif_block_bytes(bb, K, B):
-    """Algorithmic bytes of one k_batch_lif launch of K steps for B live flies: per step, the LIF state traffic
+    """Algorithmic bytes of one k_batch_lif launch of K steps (a mean over launches may be fractional) for B live
+    flies: per step, the LIF state traffic
     B*N_act*(32 + 8 + 1/8) and the synaptic input 8 * sum of out-degrees over the presynaptic cells that fired in
     any fly + B*N_act/8 for the spike bits (the per-step counts of the measurement plan).  The firing set is
     measured over the steps held in the ring."""
@@ -82,8 +84,9 @@ def lif_block_bytes(bb, K, B):
     return K * per_step, act / (B * N)
 
 
-def time_lif_launches(bb, n=6):
-    """Average duration (ms) of k_batch_lif launches of n_delay steps, with HIP events on the launch stream."""
+def time_lif_launches(bb):
+    """The k_batch_lif launches of one frame (every offset, as the captured frame issues them), each timed with HIP
+    events on its launch stream.  Returns (mean duration in ms, mean steps per launch)."""
     import ctypes as ct
     import torch
     from flyhip.batch import _lib
@@ -92,11 +95,10 @@ def time_lif_launches(bb, n=6):
     arr = (ct.c_void_p * len(lif))(*(t.data_ptr() for t in lif))
     stream = torch.cuda.current_stream()
     fn = _lib(bb.device).launch_batch_lif
-    times = []
-    # launches at offsets 0, n_delay, ... of the next frame; the state just advances (the bench is over)
-    K = bb.steps_per_launch
-    for i in range(n):
-        off = (i % max(1, bb.frame_steps // K)) * K
+    times, ks = [], []
+    # the state just advances by one frame's LIF steps (the timed region is over)
+    for off in range(0, bb.frame_steps, bb.steps_per_launch):
+        K = min(bb.steps_per_launch, bb.frame_steps - off)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         err = fn(arr, bb._dims, off, K, stream.cuda_stream)
@@ -105,7 +107,22 @@ def time_lif_launches(bb, n=6):
             raise RuntimeError(f"launch_batch_lif: {err}")
         e1.synchronize()
         times.append(e0.elapsed_time(e1))
-    return float(np.mean(times[1:])) if len(times) > 1 else times[0]
+        ks.append(K)
+    return float(np.mean(times)), float(np.mean(ks))
+
+
+def measured_traffic(shape):
+    """HBM bytes per k_batch_lif launch at launch shape `shape` (fold mode, steps per launch, workgroups per CU) from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, or None if it holds no figure for that shape."""
+    path = ROOT / "profiles" / "hbm_traffic_k_batch_lif_b64.txt"
+    if shape is None or not path.exists():
+        return None
+    key = f"{shape['fold_mode']},{shape['steps_per_launch']},{shape['grid_per_cu']}"
+    for line in path.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == key:
+            return int(f[2])
+    return None
 
 
 def optic_training(steps=3, warmup=1):
@@ -171,7 +188,7 @@ def packed_run(steps, log):
     ms = e0.elapsed_time(e1)
     sim = pb.k * steps * lp.dt / 1000.0 / (ms / 1000.0)
     rate = float(pb.counts.sum().item()) / pb.n / (steps * lp.dt / 1000.0)
-    log(f"packed: {steps} steps in one launch, {ms:.1f} ms ({ms * 1000 / steps:.1f} us/step): "
+    log(f"packed: {steps} steps in one graph launch, {ms:.1f} ms ({ms * 1000 / steps:.1f} us/step): "
         f"{sim:.2f} instance-sim-s/wall-s, mean rate {rate:.1f} Hz")
     out = {"packed_subgraphs": pb.k, "packed_cells": pb.n, "packed_edges": pb.nnz, "packed_steps": steps,
            "packed_ms": round(ms, 2), "packed_us_per_step": round(ms * 1000 / steps, 2),
@@ -217,6 +234,8 @@ def main():
     ap.add_argument("--packed-steps", type=int, default=5000)
     ap.add_argument("--no-packed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape of the B=64 engine")
+    ap.add_argument("--b64-shape", default=None, metavar="FOLD,K,GRID_PER_CU",
+                    help="pin the B=64 launch shape (fold mode, steps per launch, workgroups per CU); skips tuning")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-optic-training", action="store_true")
     ap.add_argument("--only-b64", action="store_true",
@@ -250,7 +269,11 @@ def main():
     bb.submit(rollouts)
     bb.prepare(graph=True)
     tuned = None
-    if not args.no_tune:
+    if args.b64_shape:
+        fm, K_, gpc = (int(x) for x in args.b64_shape.split(","))
+        tuned = {"fold_mode": fm, "steps_per_launch": K_, "grid_per_cu": gpc}
+        bb.configure(**tuned)
+    elif not args.no_tune:
         from flyhip.autotune import tune_batch
         tuned = tune_batch(bb, use_cache=False, log=log)["config"]
         log(f"B=64 launch shape: {tuned}")
@@ -270,12 +293,15 @@ def main():
     log(f"B=64: {args.steps} frames, {sim_steps} fly-steps in {gpu_ms:.1f} ms (host wall {wall*1e3:.1f} ms): "
         f"{value:.3f} sim-s/wall-s; rollouts completed {bb.completed}")
 
-    K = bb.steps_per_launch
-    lif_ms = time_lif_launches(bb)
+    lif_ms, K = time_lif_launches(bb)
     alg_bytes, activity = lif_block_bytes(bb, K, 64)
     achieved = alg_bytes / (lif_ms / 1e3) / 1e9
-    log(f"k_batch_lif: {lif_ms:.3f} ms per {K}-step launch, activity {activity*100:.3f} % per step, "
-        f"{alg_bytes/1e6:.1f} MB algorithmic -> {achieved:.0f} GB/s")
+    log(f"k_batch_lif: {lif_ms:.3f} ms per launch ({K:.3f} steps on average), activity {activity*100:.3f} % per "
+        f"step, {alg_bytes/1e6:.1f} MB algorithmic -> {achieved:.0f} GB/s")
+    if tuned is None:
+        tuned = {"fold_mode": bb.fold_mode, "steps_per_launch": bb.steps_per_launch,
+                 "grid_per_cu": bb.grid // torch.cuda.get_device_properties(dev).multi_processor_count}
+    traffic = measured_traffic(tuned)
     del bb
     torch.cuda.empty_cache()
     if args.only_b64:
@@ -345,8 +371,8 @@ def main():
                    "b1_ms_per_frame": round(b1_ms / args.b1_frames, 4), "spike_activity_per_step": activity,
                    "b64_launch_shape": tuned, **(ol or {}), **(packed or {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_batch_lif", "launch_ms": round(lif_ms, 4), "steps_per_launch": K,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_batch_lif", "launch_ms": round(lif_ms, 4), "steps_per_launch": round(K, 4),
                      "algorithmic_bytes_per_launch": int(alg_bytes)},
         "cpu_baseline": cpu,
     }
