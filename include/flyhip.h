@@ -83,6 +83,116 @@ int launch_feedback(void** p, int n, int n_spk, int batch, void* stream);
 /* drive[:, spk_idx] = clamp(gain_out * y_sr, -clip, clip), flyverse/optic.py:602.  p = [y_sr, spk_idx, drive]. */
 int launch_drive_out(void** p, int n, int n_spk, int batch, float gain_out, float clip, void* stream);
 
+/* launch_lif_rng that also writes the transmitted 0/1 spikes as packed bits: p[17] = uint64 words (batch, ceil(n/64)),
+ * bit i % 64 of word i / 64 set when cell i fired (no short-term depression).  With launch_pull_bits it replaces
+ * the delay-ring write spike_buf[pos] = spike of flyverse/brain.py:864-865 for the plain single-fly path. */
+int launch_lif_bits(void** p, int n, int batch, int flags, uint64_t seed, uint32_t step, void* stream);
+
+/* launch_pull over packed spike bits: p = [ptr(n+1), idx, w, bits uint64 (batch, ceil(n/64)), g (batch, n),
+ * sched (n: rows with in-edges first, longest first, then the rest)]; n_full = rows with in-edges; grid =
+ * workgroups of the persistent launch.  Replaces g.mul_(a_s) + event_scatter, flyverse/brain.py:875-876. */
+int launch_pull_bits(void** p, int n, int n_full, int batch, float a_s, int flags, int grid, void* stream);
+
+/* ------------------------------------------------------------------ batched engine: 64 flies, fly-minor (rows, 64)
+ * State is (rows, 64): lane b of row r is fly b's copy of cell r.  Spikes are 64-bit masks (one bit per fly) in a
+ * ring of ring_slots >= 2 * n_delay entries.  One launch runs K <= n_delay LIF steps of every row.  Together these
+ * replace BatchSim.step / FlyBrain._graph_frame over B flies (flyverse/batch_sim.py:203-254, flyverse/fly.py:509-610)
+ * with per-fly step counters, so rollouts of uneven length share one captured graph. */
+
+/* K LIF steps (global steps *engine_step + step_offset ...).  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v,
+ * g, refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12)];
+ * dims = [n_rows, lanes, ring_slots, delay, record, grid, fold mode 0|1|2, row_major ring]. */
+int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream);
+
+/* y (rows, 64) = scale * W x (cols, 64), chunked order of launch_csr.  p = [ptr, idx, w, sched, x, y].
+ * Replaces the per-frame optic products W_rp@a_pr, W_rs@s, W_sr@dr (flyverse/optic.py:579-580, 602). */
+int launch_spmm_fm(void** p, int rows, float scale, int grid, void* stream);
+
+/* One optic substep for 64 flies: y = W_rr dr_cur fused with the update of launch_optic; writes dr_next.
+ * p = [ptr, idx, w, sched, dr_cur, dr_next, v, adapt, pr, spk, a, b, live_mask].  Replaces OpticLobe._substep
+ * (flyverse/optic.py:525-551). */
+int launch_optic_substep_fm(void** p, int rows, float gr, float ga, float aad, int grid, void* stream);
+
+/* dr = clip(v + b, 0, 1) - b on (rows, 64).  p = [v, b, dr].  launch_optic_dr in the fly-minor layout. */
+int launch_optic_dr_fm(void** p, int rows, void* stream);
+
+/* Photoreceptor stage with the movie drawn in-kernel (Philox, stream 1).  p = [sens, I_lp, I_mean, a_pr,
+ * movie_seed, frame, fresh_mask, live_mask, coefs].  launch_photoreceptor in the fly-minor layout. */
+int launch_photoreceptor_fm(void** p, int n_pr, void* stream);
+
+/* s = clamp(rate / 100, 0, 3) for the spiking rows.  p = [rate (rows, 64), spk_row (n_spk), s (n_spk, 64)]. */
+int launch_feedback_fm(void** p, int n_spk, void* stream);
+
+/* drive[spk_row] = clamp(gain_out * y_sr, -clip, clip).  p = [y_sr (n_spk, 64), spk_row, drive (rows, 64)]. */
+int launch_drive_out_fm(void** p, int n_spk, float gain_out, float clip, void* stream);
+
+/* End of a frame: engine step += steps, live flies' step / budget / frame counters advance.
+ * p = [engine_step, fly_step, budget, frame, fresh_mask, live_mask]. */
+int launch_frame_end(void** p, int lanes, int steps, void* stream);
+
+/* Reset the lanes in `mask` to rest (a fly finished and a new rollout takes its lane; flyverse/brain.py:719-733,
+ * flyverse/fly.py:682-707) and set the live mask to live_set.  p = [v, g, refrac, rate, counts, drive, pois,
+ * optic v, optic adapt, ring, fresh_mask, live_mask]. */
+int launch_reset_lanes(void** p, uint64_t mask, uint64_t live_set, float v_rest, int n_rows, int n_rate,
+                       int ring_slots, void* stream);
+
+/* ------------------------------------------------------------------ packed uneven subgraphs
+ * Many LIF graphs packed block-diagonally into one CSR with offset tables (instance / local index per cell, one
+ * Philox key per instance).  The reference would run one Brain per Connectome.subset (flyverse/connectome.py:469-481).
+ * p = [ptr, idx, w, sched, v, g, refrac, rate, counts, drive, pois, inst, local, seeds uint64, ring uint64 (64,
+ * ceil(n/64)), bar (2 uint32), error (4 int32), P(12)]. */
+
+/* `steps` steps from step t0 in one cooperative launch; dims = [n, steps, window, delay, grid, t0, barrier bound ms];
+ * window <= delay, 2*delay + window <= 64, grid <= packed_max_grid. */
+int launch_packed_run(void** p, const int* dims, void* stream);
+
+/* One window of K <= window steps from step *t_base + off (t_base: device int32, so a captured graph of windows
+ * replays at advancing steps); dims = [n, K, window, delay, grid, off]. */
+int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream);
+
+/* *t_base += steps on the stream. */
+int launch_packed_advance(int* t_base, int steps, void* stream);
+
+/* The largest grid launch_packed_run accepts. */
+int packed_max_grid(int* out);
+
+/* ------------------------------------------------------------------ optic-lobe adjoint (32 movies, fly-minor)
+ * Forward and hand-written backward of the optic rate network for fitting per-type tau, per-type bias and type-pair
+ * gains.  The reference has no adjoint: torch autograd through OpticLobe._step_frame_grad (flyverse/optic.py:606-676)
+ * with tau_by_type / baseline_by_type / pair_gain as the parameters (flyverse/optic.py:82-87, 251-267, 343-347). */
+
+/* a = exp(-dt / tau[type]), b = bias[type].  p = [tau, bias, unit_type (n), a (n), b (n)]. */
+int launch_adj_params(void** p, int n, float dt, void* stream);
+
+/* w = gain[pair_bin] * w0.  p = [gain (types*types), pair_bin (nnz), w0 (nnz), w (nnz)]. */
+int launch_adj_weights(void** p, int64_t nnz, void* stream);
+
+/* dr = clip(v + b, 0, 1) - b on (n, 32).  p = [v, b, dr]. */
+int launch_adj_dr(void** p, int n, void* stream);
+
+/* One forward step.  p = [ptr, idx, sched, w, a, b, v, adapt, dr, P, v1, adapt1, dr1, inp|null, readout,
+ * target|null, loss|null]; f = [g_rr, g_ad, a_ad]. */
+int launch_adj_forward(void** p, int rows, const float* f, int grid, void* stream);
+
+/* Adjoint through v_t+1 = inp + (v - inp)*a: ly = g_rr * lv * (1 - a) (the adjoint of y = W dr), and
+ * ga[r] += sum over movies of lv * (v - inp) (the partial for a_r, in a fixed butterfly order).
+ * p = [lv, v, inp, a, ly, ga]. */
+int launch_adj_back_ly(void** p, int rows, float g_rr, void* stream);
+
+/* Adjoint of one step through the transpose CSR (W^T ly) and the clip gates.  p = [ptrT, idxT, schedT, wT, a, b, ly,
+ * v, dr, lv, la, readout, target|null, gb]; f = [g_ad, a_ad, loss_scale]. */
+int launch_adj_back_rows(void** p, int rows, const float* f, int grid, void* stream);
+
+/* Per-edge gain partials eg[e] += sum over movies of ly[row] * dr[col].  p = [ptr, idx, ly, dr, eg]. */
+int launch_adj_back_edges(void** p, int rows, int grid, void* stream);
+
+/* out[s] = fixed-order sum over segment s of vals[perm[j]] (* mult[perm[j]]).  p = [vals, mult|null, perm,
+ * seg (nseg+1), out]. */
+int launch_segment_sum(void** p, int nseg, void* stream);
+
+/* out[0] = scale * fixed-order sum of n partial losses.  p = [parts, out]. */
+int launch_loss_total(void** p, int64_t n, float scale, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

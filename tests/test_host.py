@@ -1,0 +1,143 @@
+"""Host-side logic on the CPU: the C ABI library exports exactly what include/flyhip.h declares, the graph builders
+and weight shaping produce what the reference's formulas give, the packed CSR is the block-diagonal of its
+instances, and every GPU entry refuses a device it cannot serve instead of falling back."""
+import ctypes as ct
+import math
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pandas as pd
+import pytest
+import scipy.sparse as sp
+
+from flyhip import hip, synth
+from flyhip.brain import Brain, LIFParams, shaped_weights
+from flyhip.connectome import Connectome
+from flyhip.packed import PackedBrains, pack_instances
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_symbols():
+    text = (ROOT / "include" / "flyhip.h").read_text()
+    return sorted(set(re.findall(r"^int\s+(\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    if not hip.LIB_PATH.exists():
+        from flyhip import build
+        build.build()
+    L = ct.CDLL(str(hip.LIB_PATH))
+    names = header_symbols()
+    assert len(names) >= 38
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, f"declared in include/flyhip.h but not exported: {missing}"
+    # and nothing exported that the header does not declare
+    out = subprocess.run(["nm", "-D", "--defined-only", str(hip.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = sorted({f.split()[-1] for f in out.splitlines() if re.search(r" T (launch_|packed_)", f)})
+    assert exported == names
+
+
+def test_reference_signatures_are_registered():
+    """The six reference entry points keep flyverse/cuda.py:73-75's argument lists."""
+    P, I, F, S = ct.POINTER(ct.c_void_p), ct.c_int, ct.c_float, ct.c_void_p
+    assert hip.SIGNATURES["launch_lif"] == [P, I, I, I, S]
+    assert hip.SIGNATURES["launch_optic"] == [P, I, I, F, F, F, S]
+    assert hip.SIGNATURES["launch_csr"] == [P, I, I, I, I, S]
+    assert hip.SIGNATURES["launch_events"] == [P, I, I, I, I, S]
+    assert hip.SIGNATURES["launch_means"] == [P, I, I, I, I, S]
+    assert hip.SIGNATURES["launch_optic_dr"] == [P, I, I, S]
+
+
+def small_connectome(rng, n=40, density=0.2):
+    W = sp.random(n, n, density=density, random_state=rng, format="csr", dtype=np.float32)
+    W.data = np.ceil(W.data * 9).astype(np.float32) * np.where(rng.random(W.nnz) < 0.7, 1, -1).astype(np.float32)
+    types = np.array([f"T{i % 5}" for i in range(n)], dtype=object)
+    sc = np.array(["cb_intrinsic" if i % 3 else "cb_sensory" for i in range(n)], dtype=object)
+    return Connectome(pd.DataFrame({"bodyId": np.arange(n) + 100, "type": types, "superclass": sc}), W)
+
+
+def test_shaped_weights_follow_the_reference_formulas():
+    """cap, same-type damping, fan-in scale and w_syn in flyverse/brain.py:302-338, 407-426 order, checked in
+    float64 against a direct restatement per entry."""
+    rng = np.random.default_rng(4)
+    c = small_connectome(rng)
+    p = LIFParams(conn_cap=5.0, same_type_gain=0.5, input_norm_alpha=0.5, input_norm_ref=20.0, path_gain=[],
+                  type_path_gain=[], w_syn=0.275)
+    W = shaped_weights(c, p).toarray().astype(np.float64)
+    A = c.W.toarray().astype(np.float64)
+    A = np.sign(A) * np.minimum(np.abs(A), 5.0)
+    t = c.neurons.type.to_numpy()
+    same = (t[:, None] == t[None, :])
+    A = np.where(same, A * 0.5, A)
+    tot = np.abs(A).sum(axis=1)
+    scale = np.clip((20.0 / np.maximum(tot, 1.0)) ** 0.5, 0.02, 1.0)
+    A = A * scale[:, None] * 0.275
+    assert np.allclose(W, A, rtol=1e-6, atol=1e-7)
+    # the Shiu setting is counts x w_syn exactly
+    Ws = shaped_weights(c, LIFParams.shiu()).toarray()
+    assert np.array_equal(Ws, (c.W.toarray() * np.float32(LIFParams.shiu().w_syn)).astype(np.float32))
+
+
+def test_plain_lif_graph_statistics():
+    """The plain-LIF workload: lognormal out-degree around 150, geometric(0.2) counts in [1, 200], signs fixed per
+    presynaptic cell with 70 % excitatory, no self-loops."""
+    c, inputs, hz = synth.plain_lif_graph(0)
+    W = c.W.tocsc()
+    n = c.n
+    assert n == 16_384 and len(inputs) == 200 and np.all(hz == 150.0)
+    assert W.diagonal().sum() == 0
+    outdeg = np.diff(W.indptr)
+    assert 120 < outdeg.mean() < 160                       # duplicates merged: slightly below the drawn 150
+    signs = np.array([np.sign(W.data[W.indptr[j]:W.indptr[j + 1]]).mean() if outdeg[j] else 1 for j in range(n)])
+    assert set(np.unique(signs)) <= {-1.0, 1.0}            # one sign per presynaptic cell
+    assert abs((signs > 0).mean() - 0.7) < 0.02
+    counts = np.abs(W.data)
+    assert counts.min() >= 1 and abs(counts.mean() - 5.0) < 0.3   # geometric(0.2) has mean 5 before merging
+
+
+def test_fly_graph_layout_at_small_scale():
+    conn, retina, L = synth.fly_graph(1, 0.02)
+    assert conn.n == L.n == len(L.central) + len(L.sensory) + len(L.pr) + len(L.rate)
+    assert len(retina.pr_index) == len(L.pr) and np.allclose(retina.pr_sens.sum(axis=1), 1.0, atol=1e-6)
+    W = conn.W.tocsr()
+    W.sort_indices()
+    assert np.all(np.diff(W.indptr) >= 0) and W.has_sorted_indices
+    assert set(conn.rate_tau) == {f"OL{i}" for i in range(len(L.rate))}
+
+
+def test_pack_instances_is_the_block_diagonal():
+    mats = synth.packed_subgraphs(2, 5, 50, 400, mean_in_degree=12.0)
+    p = LIFParams.shiu()
+    indptr, indices, data = pack_instances(mats, p)
+    blocks = []
+    for m in mats:
+        n = m.shape[0]
+        c = Connectome(pd.DataFrame({"bodyId": np.arange(n), "type": [""] * n, "superclass": ["cb_intrinsic"] * n}),
+                       sp.csr_matrix(m))
+        blocks.append(shaped_weights(c, p))
+    ref = sp.block_diag(blocks, format="csr", dtype=np.float32)
+    ref.sum_duplicates()
+    ref.sort_indices()
+    assert np.array_equal(indptr, ref.indptr) and np.array_equal(indices, ref.indices)
+    assert np.array_equal(data, ref.data)
+
+
+def test_gpu_entry_points_refuse_the_cpu():
+    rng = np.random.default_rng(0)
+    c = small_connectome(rng)
+    with pytest.raises(ValueError):
+        Brain(c, LIFParams.shiu(), device="cpu")
+    with pytest.raises(ValueError):
+        PackedBrains([c.W], device="cpu")
+    with pytest.raises(ValueError):
+        hip.use("cpu", explicit=True)
+    assert hip.use("cpu") is False
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr
