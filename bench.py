@@ -111,6 +111,37 @@ def time_lif_launches(bb):
     return float(np.mean(times)), float(np.mean(ks))
 
 
+def spmv_roofline(conn, retina, op, reps=50):
+    """The optic recurrent product y = W_rr x (flyverse/optic.py:474-483) at B=1 on the full-scale graph, grouped
+    kernel (hip.CSR default): mean duration over `reps` launches timed with HIP events on the launch stream, and
+    algorithmic bytes 8 nnz + 4 (rows + 1) + 4 (cols + rows)."""
+    import torch
+    from flyhip import hip
+    from flyhip.optic import optic_matrices
+    M = optic_matrices(conn, retina, op)["M_rr"]
+    out = {}
+    for kernel in ("grouped", "chunked"):
+        csr = hip.CSR(M, "cuda", kernel=kernel)
+        x = torch.rand(1, M.shape[1], device="cuda")
+        y = csr.matvec(x)
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            csr.matvec(x, out=y)
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1000 / reps
+        alg = 8 * M.nnz + 4 * (M.shape[0] + 1) + 4 * (M.shape[0] + M.shape[1])
+        out[kernel] = (us, alg / (us * 1e-6) / 1e9)
+        del csr
+    us, gbs = out["grouped"]
+    return {"spmv_wrr_b1_us": round(us, 2), "spmv_wrr_b1_gbs": round(gbs, 1),
+            "spmv_wrr_b1_frac": round(gbs / HBM_PEAK_GBS, 4), "spmv_wrr_nnz": int(M.nnz),
+            "spmv_wrr_b1_two_pass_us": round(out["chunked"][0], 2)}
+
+
 def measured_traffic(shape):
     """HBM bytes per k_batch_lif launch at launch shape `shape` (fold mode, steps per launch, workgroups per CU) from
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, or None if it holds no figure for that shape."""
@@ -229,6 +260,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60, help="timed 10 ms frames of the B=64 engine")
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tune-warm", type=int, default=30, help="frames run before the tuning trials")
     ap.add_argument("--b1-frames", type=int, default=30)
     ap.add_argument("--cpu-frames", type=int, default=60)
     ap.add_argument("--packed-steps", type=int, default=5000)
@@ -275,7 +307,7 @@ def main():
         bb.configure(**tuned)
     elif not args.no_tune:
         from flyhip.autotune import tune_batch
-        tuned = tune_batch(bb, use_cache=False, log=log)["config"]
+        tuned = tune_batch(bb, use_cache=False, log=log, warm=args.tune_warm)["config"]
         log(f"B=64 launch shape: {tuned}")
     bb.run(args.warmup)
     torch.cuda.synchronize()
@@ -309,6 +341,10 @@ def main():
             copy_calibration(dev)
         print(json.dumps({"b64_sim_s_per_wall_s": round(value, 4), "lif_launch_ms": round(lif_ms, 4)}), flush=True)
         return
+
+    spmv = spmv_roofline(conn, retina, op)
+    log(f"W_rr product, B=1: {spmv['spmv_wrr_b1_us']} us, {spmv['spmv_wrr_b1_gbs']} GB/s algorithmic "
+        f"({spmv['spmv_wrr_b1_frac'] * 100:.1f} % of peak); two-pass kernel {spmv['spmv_wrr_b1_two_pass_us']} us")
 
     # ---------------------------------------------------------------- B = 1, plain kernels
     log("B=1 plain path: setup")
@@ -369,7 +405,7 @@ def main():
                    "batch": 64, "neurons": int(conn.n), "edges": int(conn.W.nnz), "dt_ms": lp.dt,
                    "b1_sim_s_per_wall_s": round(b1, 4), "ratio_b64_vs_b1": round(value / b1, 2),
                    "b1_ms_per_frame": round(b1_ms / args.b1_frames, 4), "spike_activity_per_step": activity,
-                   "b64_launch_shape": tuned, **(ol or {}), **(packed or {})},
+                   "b64_launch_shape": tuned, **spmv, **(ol or {}), **(packed or {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_batch_lif", "launch_ms": round(lif_ms, 4), "steps_per_launch": round(K, 4),

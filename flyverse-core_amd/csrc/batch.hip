@@ -162,6 +162,27 @@ __device__ __forceinline__ float fold_edges_transposed(float acc, const uint32_t
     return acc;
 }
 
+// fold_chunk_transposed with the weights read by lane permute instead of staged in LDS: lane j holds the weight of
+// edge j, and the lane of fly b fetches the weight of its next set bit with ds_bpermute.
+__device__ __forceinline__ float fold_chunk_permute(float acc, uint64_t mj, float wj, int lane) {
+    uint32_t lo = (uint32_t)mj, hi = (uint32_t)(mj >> 32);
+    const uint32_t send = lane < 32 ? hi : lo;
+    const uint32_t got = (uint32_t)__shfl_xor((int)send, 32);
+    if (lane < 32) hi = got; else lo = got;
+    lo = transpose32(lo, lane);
+    hi = transpose32(hi, lane);
+    while (__builtin_amdgcn_ballot_w64((lo | hi) != 0u)) {
+        const bool any = (lo | hi) != 0u;
+        const int j = lo ? __builtin_ctz(lo) : (hi ? 32 + __builtin_ctz(hi) : 0);
+        const float wv = __int_as_float(__builtin_amdgcn_ds_bpermute(j << 2, __float_as_int(wj)));
+        if (any) {
+            acc = acc + wv;
+            if (lo) lo &= lo - 1u; else hi &= hi - 1u;
+        }
+    }
+    return acc;
+}
+
 struct BatchLif {
     const int* ptr;          // (n_rows + 1) post-major CSR over LIF rows
     const uint32_t* pre_off; // (nnz) byte offset of the presynaptic row's masks in the ring (row * ring_slots * 8)
@@ -182,6 +203,7 @@ struct BatchLif {
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, int K) {
     __shared__ float wstage[4][64];
     const int lane = threadIdx.x & 63;
@@ -210,18 +232,9 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
         float cnt = a.record ? a.counts[q] : 0.0f;
         const float drive = a.drive[q];
         const float p = has_pois ? a.pois[q] : 0.0f;
-        for (int k = 0; k < K; ++k) {
+        // LIF update of flyverse/brain.py:838-868 with active = 1, no adaptation or depression, for step k
+        auto lif_step = [&](int k, float acc) {
             const uint32_t t = T0 + (uint32_t)k;
-            const uint32_t rslot = (t + R - D) % R;
-            const char* rb = (const char*)(a.row_major ? a.ring + rslot : a.ring + (size_t)rslot * a.n_rows);
-            float acc = g * a_s;
-            if (a.mode == 2)
-                acc = fold_edges_transposed(acc, a.pre_off, a.w, rb, e0, e1, lane, wstage[threadIdx.x >> 6]);
-            else if (a.mode == 1)
-                acc = fold_edges_gather(acc, a.pre_off, (cf32p)a.w, rb, e0, e1, lane);
-            else
-                acc = fold_edges(acc, (cu32p)a.pre_off, (cf32p)a.w, rb, e0, e1);
-            // LIF update of flyverse/brain.py:838-868 with active = 1, no adaptation or depression
             float target = v_rest + acc;
             target = target + drive;
             float dv = v - target;
@@ -244,6 +257,99 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
             }
             const uint64_t bits = __builtin_amdgcn_ballot_w64(live && fired);
             if (lane == 0) a.ring[a.row_major ? (size_t)r * R + t % R : (size_t)(t % R) * a.n_rows + r] = bits;
+        };
+        auto slot_of = [&](int k) {
+            const uint32_t rslot = (T0 + (uint32_t)k + R - D) % R;
+            return (const char*)(a.row_major ? a.ring + rslot : a.ring + (size_t)rslot * a.n_rows);
+        };
+        if (MODE == 3) {
+            // Work items (step k, group of 4 chunks of 64 edges), groups inner.  The mask gathers of item i + 1 are
+            // issued before item i is folded (two mask buffers that swap roles, so no load is waited for early).
+            // Rows of at most 8 chunks keep their offsets and weights in registers for the whole launch; longer
+            // rows load those of item i + 1 just before its gathers.
+            const int ne = e1 - e0;
+            if (ne == 0) {
+                for (int k = 0; k < K; ++k) lif_step(k, g * a_s);
+            } else {
+                const int G = (ne + 255) >> 8;
+                const int items = K * G;
+                // raw mask words; lanes past the row's end are cleared when folded, so nothing waits on the loads here
+                auto gather = [&](int it, const uint32_t* o, uint64_t* m) {
+                    const char* rb = slot_of(it / G);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) m[u] = *(const uint64_t*)(rb + o[u]);
+                };
+                float acc = 0.0f;
+                auto fold = [&](int it, const uint64_t* m, const float* wv) {
+                    const int grp = it % G;
+                    if (grp == 0) acc = g * a_s;
+                    const int nch = min(4, (ne - grp * 256 + 63) >> 6);
+                    const int base = grp * 256 + lane;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (u < nch) acc = fold_chunk_permute(acc, base + u * 64 < ne ? m[u] : 0ull, wv[u], lane);
+                    if (grp == G - 1) lif_step(it / G, acc);
+                };
+                // the prefetches are unconditional (past the last item they repeat it) so the wait before a fold
+                // counts only the gathers issued after its own
+                auto run = [&](auto item_ow) {
+                    uint32_t oA[4], oB[4];
+                    float wA[4], wB[4];
+                    uint64_t mA[4], mB[4];
+                    const int last = items - 1;
+                    item_ow(0, oA, wA);
+                    gather(0, oA, mA);
+                    for (int it = 0; it < items; it += 2) {
+                        item_ow(min(it + 1, last), oB, wB);
+                        gather(min(it + 1, last), oB, mB);
+                        fold(it, mA, wA);
+                        item_ow(min(it + 2, last), oA, wA);
+                        gather(min(it + 2, last), oA, mA);
+                        if (it + 1 < items) fold(it + 1, mB, wB);
+                    }
+                };
+                if (G <= 2) {
+                    // offsets and weights of the row's (at most 8) chunks stay in registers for the whole launch
+                    uint32_t oc[8];
+                    float wc[8];
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int e = min(e0 + c * 64 + lane, e1 - 1);
+                        oc[c] = a.pre_off[e];
+                        wc[c] = a.w[e];
+                    }
+                    run([&](int it, uint32_t* o, float* wv) {
+                        const bool hi = (it % G) != 0;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            o[u] = hi ? oc[4 + u] : oc[u];
+                            wv[u] = hi ? wc[4 + u] : wc[u];
+                        }
+                    });
+                } else {
+                    run([&](int it, uint32_t* o, float* wv) {
+                        const int base = e0 + (it % G) * 256 + lane;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int e = min(base + u * 64, e1 - 1);
+                            o[u] = a.pre_off[e];
+                            wv[u] = a.w[e];
+                        }
+                    });
+                }
+            }
+        } else {
+            for (int k = 0; k < K; ++k) {
+                const char* rb = slot_of(k);
+                float acc = g * a_s;
+                if (MODE == 2)
+                    acc = fold_edges_transposed(acc, a.pre_off, a.w, rb, e0, e1, lane, wstage[threadIdx.x >> 6]);
+                else if (MODE == 1)
+                    acc = fold_edges_gather(acc, a.pre_off, (cf32p)a.w, rb, e0, e1, lane);
+                else
+                    acc = fold_edges(acc, (cu32p)a.pre_off, (cf32p)a.w, rb, e0, e1);
+                lif_step(k, acc);
+            }
         }
         a.v[q] = v; a.g[q] = g; a.refrac[q] = ref; a.rate[q] = rate;
         if (a.record) a.counts[q] = cnt;
@@ -458,9 +564,10 @@ inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
 
 // Up to `delay` LIF steps of every row for 64 flies.  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v, g,
 // refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12)];
-// dims = [n_rows, lanes, ring_slots, delay, record, grid, mode, row_major]; mode 0 reads the spike masks through the scalar
-// cache, mode 1 gathers them with vector loads, mode 2 gathers and transposes them (fold_edges,
-// fold_edges_gather, fold_edges_transposed: the same additions in the same order, so the same result).
+// dims = [n_rows, lanes, ring_slots, delay, record, grid, mode, row_major]; mode 0 reads the spike masks through the
+// scalar cache, mode 1 gathers them with vector loads, mode 2 gathers and transposes them, mode 3 transposes with
+// the gathers of the next work item in flight (fold_edges, fold_edges_gather, fold_edges_transposed,
+// fold_chunk_permute: the same additions in the same order, so the same result).
 FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream) {
     BatchLif a;
     a.ptr = (const int*)p[0]; a.pre_off = (const uint32_t*)p[1]; a.w = (const float*)p[2]; a.sched = (const int*)p[3];
@@ -474,10 +581,15 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     a.mode = dims[6];
     a.row_major = dims[7];
     if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
-        grid <= 0)
+        grid <= 0 || a.mode < 0 || a.mode > 3)
         return FH_ERR_ARGS;
     if (a.n_rows == 0 || K == 0) return 0;
-    hipLaunchKernelGGL(k_batch_lif, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
+    switch (a.mode) {
+        case 0: hipLaunchKernelGGL(k_batch_lif<0>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
+        case 1: hipLaunchKernelGGL(k_batch_lif<1>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
+        case 2: hipLaunchKernelGGL(k_batch_lif<2>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
+        default: hipLaunchKernelGGL(k_batch_lif<3>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
+    }
     FH_LAUNCH_CHECK();
 }
 

@@ -74,6 +74,80 @@ __global__ __launch_bounds__(256) void k_chunk_fold(const int* __restrict__ chun
     y[t] = total;
 }
 
+// Single-pass grouped product over chunk-padded storage: every row's entries are stored padded to whole chunks
+// (pad entries have idx -1 and are skipped, so they add nothing, not even 0 * x), chunk c of the matrix occupying
+// entries [16 c, 16 c + 16).  Consecutive rows are grouped on the host so a group has at most 64 chunks and 64 rows;
+// one wavefront per (group, fly): lane j forms the sum of the group's chunk j from two 64-byte vector loads, the sums
+// go through LDS, and lane r folds row r's chunk sums in order.  A row of more than 64 chunks is a group of its own
+// and its wavefront folds the chunk sums of each round of 64 in order.  Same additions, same order as k_csr.
+__global__ __launch_bounds__(256) void k_spmv_groups(const int* __restrict__ group_row, const int* __restrict__ chunk_ptr,
+                                                     const int4* __restrict__ pidx, const float4* __restrict__ pw,
+                                                     const float* __restrict__ x, float* __restrict__ y, int ngroups,
+                                                     int rows, int cols, float scale, int accumulate) {
+    __shared__ float sums[4][kWave];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = blockIdx.x * 4 + wv;
+    if (g >= ngroups) return;
+    const int b = blockIdx.y;
+    const float* xb = x + (int64_t)b * cols;
+    const int r0 = group_row[g], r1 = group_row[g + 1];
+    const int c0 = chunk_ptr[r0], c1 = chunk_ptr[r1];
+    auto chunk_sum = [&](int c) {
+        const int4* ip = pidx + (int64_t)c * 4;
+        const float4* wp = pw + (int64_t)c * 4;
+        int4 iv[4];
+        float4 wv4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { iv[q] = ip[q]; wv4[q] = wp[q]; }
+        float xv[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xv[4 * q + 0] = xb[max(iv[q].x, 0)];
+            xv[4 * q + 1] = xb[max(iv[q].y, 0)];
+            xv[4 * q + 2] = xb[max(iv[q].z, 0)];
+            xv[4 * q + 3] = xb[max(iv[q].w, 0)];
+        }
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float c_;
+            c_ = wv4[q].x * xv[4 * q + 0]; if (iv[q].x >= 0) s = s + c_;
+            c_ = wv4[q].y * xv[4 * q + 1]; if (iv[q].y >= 0) s = s + c_;
+            c_ = wv4[q].z * xv[4 * q + 2]; if (iv[q].z >= 0) s = s + c_;
+            c_ = wv4[q].w * xv[4 * q + 3]; if (iv[q].w >= 0) s = s + c_;
+        }
+        return s;
+    };
+    auto finish = [&](float total, int row) {
+        if (scale != 1.0f) total = total * scale;
+        float* yp = y + (int64_t)b * rows + row;
+        if (accumulate) total = *yp + total;
+        *yp = total;
+    };
+    if (c1 - c0 <= kWave) {
+        const float s = lane < c1 - c0 ? chunk_sum(c0 + lane) : 0.0f;
+        sums[wv][lane] = s;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < r1 - r0) {
+            const int row = r0 + lane;
+            float total = 0.0f;
+            for (int c = chunk_ptr[row] - c0; c < chunk_ptr[row + 1] - c0; ++c) total = total + sums[wv][c];
+            finish(total, row);
+        }
+    } else {                                       // one long row
+        float total = 0.0f;
+        for (int cb = c0; cb < c1; cb += kWave) {
+            const float s = cb + lane < c1 ? chunk_sum(cb + lane) : 0.0f;
+            const int n = min(kWave, c1 - cb);
+            for (int j = 0; j < n; ++j) total = total + fh_readlane(s, j);
+        }
+        if (lane == 0) finish(total, r0);
+    }
+}
+
 }  // namespace
 
 // Same contract as flyverse/kernels/neural.cu:105 (launch_csr): p = [ptr int32, idx int32, values (float or half),
@@ -89,6 +163,20 @@ FH_EXPORT int launch_csr(void** p, int rows, int cols, int batch, int half, void
     else
         hipLaunchKernelGGL(k_csr<float>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
                            (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], rows, cols, batch);
+    FH_LAUNCH_CHECK();
+}
+
+// Single-pass grouped product, same order as launch_csr, over chunk-padded storage: p = [group_row (ngroups+1):
+// first row of each group, chunk_ptr (rows+1), pidx int32 (16 * nchunks, pad = -1), pw float (16 * nchunks),
+// x (B, cols), y (B, rows)].  y = scale * (W x) (+ y if accumulate).  Groups hold at most 64 chunks and 64 rows,
+// or one row of more than 64 chunks.
+FH_EXPORT int launch_spmv_grouped(void** p, int rows, int cols, int ngroups, int batch, float scale, int accumulate,
+                                  void* stream) {
+    if (rows < 0 || cols < 0 || batch < 0 || ngroups < 0 || batch > 65535) return FH_ERR_ARGS;
+    if ((int64_t)rows * batch == 0 || ngroups == 0) return 0;
+    hipLaunchKernelGGL(k_spmv_groups, dim3((unsigned)((ngroups + 3) / 4), (unsigned)batch), dim3(256), 0,
+                       (hipStream_t)stream, (const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3],
+                       (const float*)p[4], (float*)p[5], ngroups, rows, cols, scale, accumulate);
     FH_LAUNCH_CHECK();
 }
 

@@ -38,10 +38,12 @@ def batch_key(bb) -> str:
     return f"batch:{dev.name}:{dev.multi_processor_count}:rows{bb.n_rows}:nnz{bb.W_lif.nnz}:rate{bb.n_rate}:D{bb.n_delay}"
 
 
-def tune_batch(bb, frames: int = 3, fold_modes=(1, 2), steps=(3, 4, 6, 9), grids=(16, 32), use_cache=True,
-               log=None) -> dict:
+def tune_batch(bb, frames: int = 3, fold_modes=(2, 3), steps=(3, 4, 6, 9), grids=(16, 32), use_cache=True,
+               log=None, warm: int = 30) -> dict:
     """Time each (fold mode, steps per launch, grid per CU) on `bb` (a BatchBrain with rollouts queued) and leave
-    it configured with the fastest.  Returns {"config": ..., "ms_per_frame": ..., "trials": [...]}."""
+    it configured with the fastest.  `warm` frames run first: flies start at rest and their firing (the work per
+    step) rises over the first tens of frames, so trials compare at a settled activity.  Returns {"config": ...,
+    "ms_per_frame": ..., "trials": [...]}."""
     key = batch_key(bb)
     table = _load() if use_cache else {}
     if key in table:
@@ -50,6 +52,8 @@ def tune_batch(bb, frames: int = 3, fold_modes=(1, 2), steps=(3, 4, 6, 9), grids
         return table[key] | {"cached": True}
     trials = []
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if warm:
+        bb.run(warm)
     for mode, K, gpc in itertools.product(fold_modes, steps, grids):
         if K > bb.n_delay:
             continue

@@ -173,18 +173,27 @@ def test_pull_half_weights():
     assert_bits(N_(g), ref, "pull half")
 
 
-@pytest.mark.parametrize("B", [1, 3, 8])
-def test_csr_products_bit_exact_and_close_to_float64(B):
+@pytest.mark.parametrize("B,kernel", [(1, "grouped"), (3, "grouped"), (8, "grouped"), (1, "chunked"), (3, "chunked")])
+def test_csr_products_bit_exact_and_close_to_float64(B, kernel):
     rng = np.random.default_rng(5 + B)
     rows, cols = 2000, 1700
     M = random_csr(rng, rows, cols, 0.03, long_rows=(0, 999, 1998), long_len=1650)
-    M = sp.vstack([M, sp.csr_matrix((3, cols), dtype=np.float32)]).tocsr()     # empty rows at the end
+    # empty rows in the middle and at the end, a block of rows of exactly 16 and 17 entries
+    dense = sp.csr_matrix(np.where(np.arange(cols)[None, :] < np.array([16, 17, 15, 1])[:, None],
+                                   rng.standard_normal((4, cols)), 0).astype(np.float32))
+    M = sp.vstack([M[:700], sp.csr_matrix((2, cols), dtype=np.float32), dense, M[700:],
+                   sp.csr_matrix((3, cols), dtype=np.float32)]).tocsr()
+    M.sort_indices()
+    rows = M.shape[0]
     x = rng.standard_normal((B, cols)).astype(np.float32)
     x[:, ::3] = 0
     ref = oracle.spmv(M.indptr, M.indices, M.data, x)
-    csr = hip.CSR(M, DEV)
+    csr = hip.CSR(M, DEV, kernel=kernel)
     y = csr.matvec(T(x))
-    assert_bits(N_(y), ref, "chunked")
+    assert_bits(N_(y), ref, kernel)
+    acc = T(rng.standard_normal((B, rows)).astype(np.float32))
+    want = (N_(acc) + (np.float32(0.5) * ref).astype(np.float32)).astype(np.float32)
+    assert_bits(N_(csr.matvec(T(x), out=acc, scale=0.5, accumulate=True)), want, "scale + accumulate")
     assert_bits(N_(csr.matvec_reference_shape(T(x))), ref, "launch_csr")
     exact = (M.astype(np.float64) @ x.astype(np.float64).T).T
     np.testing.assert_allclose(N_(y), exact, rtol=1e-5, atol=1e-5 * np.abs(exact).max())
