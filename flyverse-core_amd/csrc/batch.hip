@@ -163,21 +163,28 @@ __device__ __forceinline__ float fold_edges_transposed(float acc, const uint32_t
 }
 
 // fold_chunk_transposed with the weights read by lane permute instead of staged in LDS: lane j holds the weight of
-// edge j, and the lane of fly b fetches the weight of its next set bit with ds_bpermute.
+// edge j, and the lane of fly b fetches the weight of its next set bit with ds_bpermute.  The half swap before the
+// two 32 x 32 transposes is one v_permlane32_swap (gfx950): lanes 0-31 get (edges 0-31, edges 32-63) of flies 0-31,
+// lanes 32-63 those of flies 32-63.  A lane's adds go over edges 0-31, then edges 32-63, each ascending.
 __device__ __forceinline__ float fold_chunk_permute(float acc, uint64_t mj, float wj, int lane) {
-    uint32_t lo = (uint32_t)mj, hi = (uint32_t)(mj >> 32);
-    const uint32_t send = lane < 32 ? hi : lo;
-    const uint32_t got = (uint32_t)__shfl_xor((int)send, 32);
-    if (lane < 32) hi = got; else lo = got;
-    lo = transpose32(lo, lane);
-    hi = transpose32(hi, lane);
-    while (__builtin_amdgcn_ballot_w64((lo | hi) != 0u)) {
-        const bool any = (lo | hi) != 0u;
-        const int j = lo ? __builtin_ctz(lo) : (hi ? 32 + __builtin_ctz(hi) : 0);
-        const float wv = __int_as_float(__builtin_amdgcn_ds_bpermute(j << 2, __float_as_int(wj)));
-        if (any) {
+    const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)mj, (uint32_t)(mj >> 32), false, false);
+    uint32_t lo = transpose32(sw[0], lane);
+    uint32_t hi = transpose32(sw[1], lane);
+    const int wi = __float_as_int(wj);
+    while (__builtin_amdgcn_ballot_w64(lo != 0u)) {
+        const int j = __builtin_ctz(lo | 0x80000000u);          // 31 (unused) once a lane is done
+        const float wv = __int_as_float(__builtin_amdgcn_ds_bpermute(j << 2, wi));
+        if (lo != 0u) {
             acc = acc + wv;
-            if (lo) lo &= lo - 1u; else hi &= hi - 1u;
+            lo &= lo - 1u;
+        }
+    }
+    while (__builtin_amdgcn_ballot_w64(hi != 0u)) {
+        const int j = 32 + __builtin_ctz(hi | 0x80000000u);
+        const float wv = __int_as_float(__builtin_amdgcn_ds_bpermute(j << 2, wi));
+        if (hi != 0u) {
+            acc = acc + wv;
+            hi &= hi - 1u;
         }
     }
     return acc;
@@ -233,8 +240,7 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
         const float drive = a.drive[q];
         const float p = has_pois ? a.pois[q] : 0.0f;
         // LIF update of flyverse/brain.py:838-868 with active = 1, no adaptation or depression, for step k
-        auto lif_step = [&](int k, float acc) {
-            const uint32_t t = T0 + (uint32_t)k;
+        auto lif_step = [&](int k, float acc, uint32_t wslot) {
             float target = v_rest + acc;
             target = target + drive;
             float dv = v - target;
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                 cnt = cnt + spike;
             }
             const uint64_t bits = __builtin_amdgcn_ballot_w64(live && fired);
-            if (lane == 0) a.ring[a.row_major ? (size_t)r * R + t % R : (size_t)(t % R) * a.n_rows + r] = bits;
+            if (lane == 0) a.ring[a.row_major ? (size_t)r * R + wslot : (size_t)wslot * a.n_rows + r] = bits;
         };
         auto slot_of = [&](int k) {
             const uint32_t rslot = (T0 + (uint32_t)k + R - D) % R;
@@ -269,26 +275,38 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
             // rows load those of item i + 1 just before its gathers.
             const int ne = e1 - e0;
             if (ne == 0) {
-                for (int k = 0; k < K; ++k) lif_step(k, g * a_s);
+                for (int k = 0; k < K; ++k) lif_step(k, g * a_s, (T0 + (uint32_t)k) % R);
             } else {
                 const int G = (ne + 255) >> 8;
-                const int items = K * G;
+                // a cursor over the work items (step k, group grp), groups inner, with the ring slots of step k
+                // (read: step t - D, write: step t) advanced by increments instead of divisions
+                struct Cursor { int k, grp; uint32_t rslot, wslot; };
+                const uint32_t r0 = (T0 + R - D) % R, w0 = T0 % R;
+                const size_t slot_bytes = a.row_major ? 8 : (size_t)a.n_rows * 8;
+                auto advance = [&](Cursor& c) {
+                    if (c.k == K - 1 && c.grp == G - 1) return;             // stays on the last item
+                    if (++c.grp == G) {
+                        c.grp = 0;
+                        ++c.k;
+                        c.rslot = c.rslot + 1 == R ? 0 : c.rslot + 1;
+                        c.wslot = c.wslot + 1 == R ? 0 : c.wslot + 1;
+                    }
+                };
                 // raw mask words; lanes past the row's end are cleared when folded, so nothing waits on the loads here
-                auto gather = [&](int it, const uint32_t* o, uint64_t* m) {
-                    const char* rb = slot_of(it / G);
+                auto gather = [&](const Cursor& c, const uint32_t* o, uint64_t* m) {
+                    const char* rb = (const char*)a.ring + (size_t)c.rslot * slot_bytes;
 #pragma unroll
                     for (int u = 0; u < 4; ++u) m[u] = *(const uint64_t*)(rb + o[u]);
                 };
                 float acc = 0.0f;
-                auto fold = [&](int it, const uint64_t* m, const float* wv) {
-                    const int grp = it % G;
-                    if (grp == 0) acc = g * a_s;
-                    const int nch = min(4, (ne - grp * 256 + 63) >> 6);
-                    const int base = grp * 256 + lane;
+                auto fold = [&](const Cursor& c, const uint64_t* m, const float* wv) {
+                    if (c.grp == 0) acc = g * a_s;
+                    const int nch = min(4, (ne - c.grp * 256 + 63) >> 6);
+                    const int base = c.grp * 256 + lane;
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
                         if (u < nch) acc = fold_chunk_permute(acc, base + u * 64 < ne ? m[u] : 0ull, wv[u], lane);
-                    if (grp == G - 1) lif_step(it / G, acc);
+                    if (c.grp == G - 1) lif_step(c.k, acc, c.wslot);
                 };
                 // the prefetches are unconditional (past the last item they repeat it) so the wait before a fold
                 // counts only the gathers issued after its own
@@ -296,16 +314,23 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                     uint32_t oA[4], oB[4];
                     float wA[4], wB[4];
                     uint64_t mA[4], mB[4];
-                    const int last = items - 1;
-                    item_ow(0, oA, wA);
-                    gather(0, oA, mA);
+                    const int items = K * G;
+                    Cursor f{0, 0, r0, w0}, p = f;                        // fold and prefetch cursors
+                    item_ow(p, oA, wA);
+                    gather(p, oA, mA);
                     for (int it = 0; it < items; it += 2) {
-                        item_ow(min(it + 1, last), oB, wB);
-                        gather(min(it + 1, last), oB, mB);
-                        fold(it, mA, wA);
-                        item_ow(min(it + 2, last), oA, wA);
-                        gather(min(it + 2, last), oA, mA);
-                        if (it + 1 < items) fold(it + 1, mB, wB);
+                        advance(p);
+                        item_ow(p, oB, wB);
+                        gather(p, oB, mB);
+                        fold(f, mA, wA);
+                        advance(f);
+                        advance(p);
+                        item_ow(p, oA, wA);
+                        gather(p, oA, mA);
+                        if (it + 1 < items) {
+                            fold(f, mB, wB);
+                            advance(f);
+                        }
                     }
                 };
                 if (G <= 2) {
@@ -318,8 +343,8 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                         oc[c] = a.pre_off[e];
                         wc[c] = a.w[e];
                     }
-                    run([&](int it, uint32_t* o, float* wv) {
-                        const bool hi = (it % G) != 0;
+                    run([&](const Cursor& c, uint32_t* o, float* wv) {
+                        const bool hi = c.grp != 0;
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             o[u] = hi ? oc[4 + u] : oc[u];
@@ -327,8 +352,8 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                         }
                     });
                 } else {
-                    run([&](int it, uint32_t* o, float* wv) {
-                        const int base = e0 + (it % G) * 256 + lane;
+                    run([&](const Cursor& c, uint32_t* o, float* wv) {
+                        const int base = e0 + c.grp * 256 + lane;
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             const int e = min(base + u * 64, e1 - 1);
@@ -348,7 +373,7 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                     acc = fold_edges_gather(acc, a.pre_off, (cf32p)a.w, rb, e0, e1, lane);
                 else
                     acc = fold_edges(acc, (cu32p)a.pre_off, (cf32p)a.w, rb, e0, e1);
-                lif_step(k, acc);
+                lif_step(k, acc, (T0 + (uint32_t)k) % R);
             }
         }
         a.v[q] = v; a.g[q] = g; a.refrac[q] = ref; a.rate[q] = rate;
