@@ -382,61 +382,94 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
 }
 
 // ------------------------------------------------------------------ optic lobe, fly-minor
-// One row of a chunked product over a (cols, 64) input: all 16 loads of a chunk are issued before its fold, so a
-// wavefront waits once per chunk instead of once per entry.
-__device__ __forceinline__ float fold_row_fm(cu32p idx, cf32p w, const float* __restrict__ x, int e0, int e1,
-                                             int lane) {
+// Chunked products over a (cols, 64) input with chunk-padded storage: row r owns chunks [chunk_ptr[r],
+// chunk_ptr[r+1]), chunk c is entries [16 c, 16 c + 16) of pidx / pw, pads have idx -1 and are skipped.  A wavefront
+// walks the chunks of its rows (sched order, stride = number of wavefronts) as one stream: the scalar loads of a
+// chunk's indices and weights and its 16 row gathers (256 bytes each, one per entry) are issued one chunk ahead, so
+// the fold of chunk c overlaps the loads of chunk c + 1.  Each chunk is folded from +0 in entry order and the chunk
+// sums of a row in chunk order (csrc/spmv.hip order); finish(row, total) consumes the row total.
+struct FmBuf {
+    uint32_t idx[16];
+    float w[16];
+    float x[16];
+};
+
+struct FmCursor { int si, r, c, ce; };
+
+template <class Finish>
+__device__ __forceinline__ void fm_chunk_stream(const int* __restrict__ chunk_ptr, cu32p pidx, cf32p pw,
+                                                const int* __restrict__ sched, const float* __restrict__ x, int rows,
+                                                int lane, Finish finish) {
+    const int nw = gridDim.x * 4;
+    // move to the first non-empty row at or after schedule index si (empty rows are finished with a zero total)
+    auto seek = [&](FmCursor& q, int si) {
+        q.si = si;
+        while (q.si < rows) {
+            q.r = sched[q.si];
+            q.c = chunk_ptr[q.r];
+            q.ce = chunk_ptr[q.r + 1];
+            if (q.c < q.ce) return;
+            finish(q.r, 0.0f);
+            q.si += nw;
+        }
+    };
+    auto next = [&](FmCursor q) {
+        if (q.c + 1 < q.ce) { ++q.c; return q; }
+        seek(q, q.si + nw);
+        return q;
+    };
+    auto load = [&](const FmCursor& q, FmBuf& b) {
+        const int c = q.c;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) { b.idx[j] = pidx[c * 16 + j]; b.w[j] = pw[c * 16 + j]; }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) b.x[j] = x[(size_t)max((int)b.idx[j], 0) * 64 + lane];
+    };
     float total = 0.0f;
-    int cs = e0;
-    for (; cs + 16 <= e1; cs += 16) {
-        uint32_t c[16];
-        float wv[16], xv[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) { c[j] = idx[cs + j]; wv[j] = w[cs + j]; }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) xv[j] = x[(size_t)c[j] * 64 + lane];
+    auto fold = [&](const FmCursor& q, const FmBuf& b) {
         float s = 0.0f;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) { float p = wv[j] * xv[j]; s = s + p; }
+        for (int j = 0; j < 16; ++j)
+            if ((int)b.idx[j] >= 0) { const float p = b.w[j] * b.x[j]; s = s + p; }
         total = total + s;
+        if (q.c + 1 == q.ce) { finish(q.r, total); total = 0.0f; }
+    };
+    FmCursor p0;
+    seek(p0, blockIdx.x * 4 + wave_id());
+    if (p0.si >= rows) return;
+    FmBuf A, B;
+    load(p0, A);
+    for (;;) {
+        const FmCursor p1 = next(p0);
+        const bool more1 = p1.si < rows;
+        load(more1 ? p1 : p0, B);                       // past the end: a repeat that is never folded
+        fold(p0, A);
+        if (!more1) break;
+        const FmCursor p2 = next(p1);
+        const bool more2 = p2.si < rows;
+        load(more2 ? p2 : p1, A);
+        fold(p1, B);
+        if (!more2) break;
+        p0 = p2;
     }
-    if (cs < e1) {
-        const int n = e1 - cs;
-        float wv[16], xv[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            if (j < n) { wv[j] = w[cs + j]; xv[j] = x[(size_t)idx[cs + j] * 64 + lane]; }
-        }
-        float s = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            if (j < n) { float p = wv[j] * xv[j]; s = s + p; }
-        }
-        total = total + s;
-    }
-    return total;
 }
 
-// Chunked CSR product over the (cols, 64) input: y[r, b] = scale * fold of chunk folds (csrc/spmv.hip order).
-__global__ __launch_bounds__(256) void k_spmm_fm(const int* __restrict__ ptr, const int* __restrict__ idx,
-                                                 const float* __restrict__ w, const int* __restrict__ sched,
+// y[r, b] = scale * row total (chunk-padded CSR, csrc/spmv.hip order) over the (cols, 64) input.
+__global__ __launch_bounds__(256) void k_spmm_fm(const int* __restrict__ chunk_ptr, const int* __restrict__ pidx,
+                                                 const float* __restrict__ pw, const int* __restrict__ sched,
                                                  const float* __restrict__ x, float* __restrict__ y, int rows,
                                                  float scale) {
     const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * 4 + wave_id();
-    const int nw = gridDim.x * 4;
-    for (int si = gw; si < rows; si += nw) {
-        const int r = sched[si];
-        float total = fold_row_fm((cu32p)idx, (cf32p)w, x, ptr[r], ptr[r + 1], lane);
+    fm_chunk_stream(chunk_ptr, (cu32p)pidx, (cf32p)pw, sched, x, rows, lane, [&](int r, float total) {
         if (scale != 1.0f) total = total * scale;
         y[(int64_t)r * 64 + lane] = total;
-    }
+    });
 }
 
 // One optic substep fused with its recurrent product: y = W_rr dr_cur (chunked order), then the update of
 // flyverse/optic.py:525-551 (neural.cu:68-79) writing the next dr into dr_next; lanes of finished flies only copy.
-__global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict__ ptr, const int* __restrict__ idx,
-                                                          const float* __restrict__ w, const int* __restrict__ sched,
+__global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict__ chunk_ptr, const int* __restrict__ pidx,
+                                                          const float* __restrict__ pw, const int* __restrict__ sched,
                                                           const float* __restrict__ dr_cur, float* __restrict__ dr_next,
                                                           float* __restrict__ v, float* __restrict__ adapt,
                                                           const float* __restrict__ pr, const float* __restrict__ spk,
@@ -444,15 +477,11 @@ __global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict_
                                                           const uint64_t* __restrict__ live_mask, int rows, float gr,
                                                           float ga, float aad) {
     const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * 4 + wave_id();
-    const int nw = gridDim.x * 4;
     const bool live = (*live_mask >> lane) & 1ull;
-    for (int si = gw; si < rows; si += nw) {
-        const int r = sched[si];
-        const float total = fold_row_fm((cu32p)idx, (cf32p)w, dr_cur, ptr[r], ptr[r + 1], lane);
+    fm_chunk_stream(chunk_ptr, (cu32p)pidx, (cf32p)pw, sched, dr_cur, rows, lane, [&](int r, float total) {
         const int64_t q = (int64_t)r * 64 + lane;
         const float old = dr_cur[q];
-        if (!live) { dr_next[q] = old; continue; }
+        if (!live) { dr_next[q] = old; return; }
         float inp = gr * total;
         inp = inp + pr[q];
         float gad = ga * adapt[q];
@@ -467,7 +496,7 @@ __global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict_
         adapt[q] = old + da;
         float s2 = vn + b[r];
         dr_next[q] = fh_clip01(s2) - b[r];
-    }
+    });
 }
 
 __global__ __launch_bounds__(256) void k_optic_dr_fm(const float* __restrict__ v, const float* __restrict__ b,
@@ -618,7 +647,7 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     FH_LAUNCH_CHECK();
 }
 
-// y (rows, 64) = scale * W x (cols, 64).  p = [ptr, idx, w, sched, x, y].
+// y (rows, 64) = scale * W x (cols, 64).  p = [chunk_ptr (rows+1), pidx (16 per chunk, pad -1), pw, sched, x, y].
 FH_EXPORT int launch_spmm_fm(void** p, int rows, float scale, int grid, void* stream) {
     if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
     if (!rows) return 0;
@@ -627,7 +656,8 @@ FH_EXPORT int launch_spmm_fm(void** p, int rows, float scale, int grid, void* st
     FH_LAUNCH_CHECK();
 }
 
-// p = [ptr, idx, w, sched, dr_cur, dr_next, v, adapt, pr, spk, a, b, live_mask].
+// p = [chunk_ptr, pidx, pw, sched, dr_cur, dr_next, v, adapt, pr, spk, a, b, live_mask] (chunk-padded CSR as
+// launch_spmm_fm).
 FH_EXPORT int launch_optic_substep_fm(void** p, int rows, float gr, float ga, float aad, int grid, void* stream) {
     if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
     if (!rows) return 0;
