@@ -190,6 +190,29 @@ __device__ __forceinline__ float fold_chunk_permute(float acc, uint64_t mj, floa
     return acc;
 }
 
+// The two halves of fold_chunk_permute, so the transposes of several chunks can be issued before their adds: the
+// transpose of one 64-edge chunk (lane b gets lo = edges 0-31, hi = edges 32-63 of fly b), then the adds of one
+// chunk over a lane's set bits, ascending, two bits per round (both weights fetched before either is added).
+__device__ __forceinline__ void transpose_chunk(uint64_t mj, int lane, uint32_t& lo, uint32_t& hi) {
+    const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)mj, (uint32_t)(mj >> 32), false, false);
+    lo = transpose32(sw[0], lane);
+    hi = transpose32(sw[1], lane);
+}
+
+__device__ __forceinline__ float add_bits(float acc, uint32_t bits, int wi, int lane_base) {
+    while (__builtin_amdgcn_ballot_w64(bits != 0u)) {
+        const uint32_t b1 = bits & (bits - 1u);
+        const int j0 = lane_base + __builtin_ctz(bits | 0x80000000u);
+        const int j1 = lane_base + __builtin_ctz(b1 | 0x80000000u);
+        const float w0 = __int_as_float(__builtin_amdgcn_ds_bpermute(j0 << 2, wi));
+        const float w1 = __int_as_float(__builtin_amdgcn_ds_bpermute(j1 << 2, wi));
+        if (bits != 0u) acc = acc + w0;
+        if (b1 != 0u) acc = acc + w1;
+        bits = b1 & (b1 - 1u);
+    }
+    return acc;
+}
+
 struct BatchLif {
     const int* ptr;          // (n_rows + 1) post-major CSR over LIF rows
     const uint32_t* pre_off; // (nnz) byte offset of the presynaptic row's masks in the ring (row * ring_slots * 8)
@@ -213,6 +236,8 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 template <int MODE>
 __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, int K) {
     __shared__ float wstage[4][64];
+    __shared__ uint64_t cmask[MODE == 4 ? 4 : 1][256];
+    __shared__ float cweight[MODE == 4 ? 4 : 1][256];
     const int lane = threadIdx.x & 63;
     const int gw = blockIdx.x * 4 + wave_id();
     const int nw = gridDim.x * 4;
@@ -268,7 +293,7 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
             const uint32_t rslot = (T0 + (uint32_t)k + R - D) % R;
             return (const char*)(a.row_major ? a.ring + rslot : a.ring + (size_t)rslot * a.n_rows);
         };
-        if (MODE == 3) {
+        if (MODE >= 3) {
             // Work items (step k, group of 4 chunks of 64 edges), groups inner.  The mask gathers of item i + 1 are
             // issued before item i is folded (two mask buffers that swap roles, so no load is waited for early).
             // Rows of at most 8 chunks keep their offsets and weights in registers for the whole launch; longer
@@ -303,9 +328,40 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                     if (c.grp == 0) acc = g * a_s;
                     const int nch = min(4, (ne - c.grp * 256 + 63) >> 6);
                     const int base = c.grp * 256 + lane;
+                    if (MODE == 4) {
+                        // the item's edges with a nonzero mask word (some fly's presynaptic cell fired), compacted
+                        // in edge order through LDS; an all-zero word adds nothing in any fly
+                        uint64_t* cm = cmask[threadIdx.x >> 6];
+                        float* cw = cweight[threadIdx.x >> 6];
+                        int n = 0;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (u < nch) acc = fold_chunk_permute(acc, base + u * 64 < ne ? m[u] : 0ull, wv[u], lane);
+                        for (int u = 0; u < 4; ++u) {
+                            const uint64_t mv = u < nch && base + u * 64 < ne ? m[u] : 0ull;
+                            const uint64_t bal = __builtin_amdgcn_ballot_w64(mv != 0ull);
+                            const int pos = n + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                            if (mv != 0ull) { cm[pos] = mv; cw[pos] = wv[u]; }
+                            n += __builtin_popcountll(bal);
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        for (int cb = 0; cb < n; cb += 64) {
+                            const uint64_t mj = cb + lane < n ? cm[cb + lane] : 0ull;
+                            acc = fold_chunk_permute(acc, mj, cw[cb + lane], lane);
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    } else {
+                        uint32_t lo[4], hi[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (u < nch) transpose_chunk(base + u * 64 < ne ? m[u] : 0ull, lane, lo[u], hi[u]);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (u < nch) {
+                                acc = add_bits(acc, lo[u], __float_as_int(wv[u]), 0);
+                                acc = add_bits(acc, hi[u], __float_as_int(wv[u]), 32);
+                            }
+                        }
+                    }
                     if (c.grp == G - 1) lif_step(c.k, acc, c.wslot);
                 };
                 // the prefetches are unconditional (past the last item they repeat it) so the wait before a fold
@@ -620,8 +676,9 @@ inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
 // refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12)];
 // dims = [n_rows, lanes, ring_slots, delay, record, grid, mode, row_major]; mode 0 reads the spike masks through the
 // scalar cache, mode 1 gathers them with vector loads, mode 2 gathers and transposes them, mode 3 transposes with
-// the gathers of the next work item in flight (fold_edges, fold_edges_gather, fold_edges_transposed,
-// fold_chunk_permute: the same additions in the same order, so the same result).
+// the gathers of the next work item in flight, mode 4 also drops the edges no fly transmits before transposing
+// (fold_edges, fold_edges_gather, fold_edges_transposed, fold_chunk_permute: the same additions in the same order,
+// so the same result).
 FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream) {
     BatchLif a;
     a.ptr = (const int*)p[0]; a.pre_off = (const uint32_t*)p[1]; a.w = (const float*)p[2]; a.sched = (const int*)p[3];
@@ -635,14 +692,15 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     a.mode = dims[6];
     a.row_major = dims[7];
     if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
-        grid <= 0 || a.mode < 0 || a.mode > 3)
+        grid <= 0 || a.mode < 0 || a.mode > 4)
         return FH_ERR_ARGS;
     if (a.n_rows == 0 || K == 0) return 0;
     switch (a.mode) {
         case 0: hipLaunchKernelGGL(k_batch_lif<0>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 1: hipLaunchKernelGGL(k_batch_lif<1>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 2: hipLaunchKernelGGL(k_batch_lif<2>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
-        default: hipLaunchKernelGGL(k_batch_lif<3>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
+        case 3: hipLaunchKernelGGL(k_batch_lif<3>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
+        default: hipLaunchKernelGGL(k_batch_lif<4>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
     }
     FH_LAUNCH_CHECK();
 }

@@ -29,7 +29,7 @@ def rollouts(layout, n, frames_lo, frames_hi, frame_steps, seed=0):
     return out
 
 
-@pytest.mark.parametrize("graph,mode", [(False, 0), (True, 1), (True, 2), (False, 2), (True, 3), (False, 3)])
+@pytest.mark.parametrize("graph,mode", [(False, 0), (True, 1), (True, 2), (False, 2), (True, 3), (False, 3), (True, 4), (False, 4)])
 def test_lanes_equal_solo_oracle_runs_with_refill(graph, mode):
     conn, retina, layout, lp, op = fly_setup(0.02, seed=2)
     frame_ms, frame_steps = 1.0, 10           # 10 LIF steps and one optic substep per frame: the ring holds a frame
@@ -65,7 +65,7 @@ def test_lanes_equal_solo_oracle_runs_with_refill(graph, mode):
 
 
 def test_full_scale_batch_is_deterministic_and_matches_plain_path():
-    """Full-scale graph, 64 flies, 3 frames of 10 ms: two runs (fold modes 2 and 3) give the same bits, and lane 7
+    """Full-scale graph, 64 flies, 3 frames of 10 ms: two runs (fold modes 2 and 4) give the same bits, and lane 7
     equals the plain single-fly FlyBrain (batch=1, same seeds and inputs) bit for bit."""
     from flyhip.brain import LIFParams
     from flyhip.fly import FlyBrain
@@ -75,7 +75,7 @@ def test_full_scale_batch_is_deterministic_and_matches_plain_path():
     op = OpticParams(tau_by_type=conn.rate_tau, baseline_by_type=conn.rate_baseline, pair_gain=[])
     rs = rollouts(layout, 64, 3, 3, 100, seed=5)
     states = []
-    for rep, mode in enumerate((2, 3)):
+    for rep, mode in enumerate((2, 4)):
         bb = BatchBrain(conn, retina, lp, op, device=DEV, fold_mode=mode)
         bb.submit(rs)
         bb.run(3)
