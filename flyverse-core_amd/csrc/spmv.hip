@@ -148,6 +148,86 @@ __global__ __launch_bounds__(256) void k_spmv_groups(const int* __restrict__ gro
     }
 }
 
+// LDS-staged product over chunk-padded storage: the gathers of x go to LDS instead of the vector caches (a 4-byte
+// gather through the vector L1 moves a whole cache line from L2).  One workgroup of 1024 threads per tile of at
+// most 1024 consecutive chunks (whole rows, host-built); thread i holds chunk i of the tile in registers.  x is
+// staged range by range (each range as many columns as the LDS buffer holds); within a range a thread adds, in
+// entry order, the entries of its chunk whose column lies in the range, so over the ranges every entry is added
+// once, in column order, which is the chunk's entry order.  Then thread i folds the chunk sums of the tile's row i
+// in chunk order.  Same additions, same order as k_csr.
+__global__ __launch_bounds__(1024) void k_spmv_lds(const int* __restrict__ tile_row, const int* __restrict__ chunk_ptr,
+                                                  const int4* __restrict__ pidx, const float4* __restrict__ pw,
+                                                  const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                  int cols, int range_cols, float scale, int accumulate) {
+    extern __shared__ float xs[];                  // range_cols floats (reused for the chunk sums)
+    const int b = blockIdx.y;
+    const float* xb = x + (int64_t)b * cols;
+    const int r0 = tile_row[blockIdx.x], r1 = tile_row[blockIdx.x + 1];
+    const int c0 = chunk_ptr[r0], c1 = chunk_ptr[r1];
+    const int t = threadIdx.x;
+    const bool mine = c0 + t < c1;
+    int iv[16];
+    float wv[16];
+    if (mine) {
+        const int4* ip = pidx + (int64_t)(c0 + t) * 4;
+        const float4* wp = pw + (int64_t)(c0 + t) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 i4 = ip[q];
+            const float4 w4 = wp[q];
+            iv[4 * q] = i4.x; iv[4 * q + 1] = i4.y; iv[4 * q + 2] = i4.z; iv[4 * q + 3] = i4.w;
+            wv[4 * q] = w4.x; wv[4 * q + 1] = w4.y; wv[4 * q + 2] = w4.z; wv[4 * q + 3] = w4.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) { iv[j] = -1; wv[j] = 0.0f; }
+    }
+    // the next range's values are loaded into registers while the current range is folded (at most 16 per thread)
+    float xr[16];
+    auto load_range = [&](int lo) {
+        const int n = min(range_cols, cols - lo);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = q * 1024 + t;
+            xr[q] = k < n ? xb[lo + k] : 0.0f;
+        }
+    };
+    float s = 0.0f;
+    load_range(0);
+    for (int lo = 0; lo < cols; lo += range_cols) {
+        const int n = min(range_cols, cols - lo);
+        __syncthreads();                           // the previous range is no longer read
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = q * 1024 + t;
+            if (k < n) xs[k] = xr[q];
+        }
+        __syncthreads();
+        if (lo + range_cols < cols) load_range(lo + range_cols);
+        const unsigned un = (unsigned)n;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const unsigned off = (unsigned)(iv[j] - lo);   // in range iff off < n (pads: idx -1 never)
+            if (iv[j] >= 0 && off < un) {
+                const float c = wv[j] * xs[off];
+                s = s + c;
+            }
+        }
+    }
+    __syncthreads();
+    xs[t] = s;
+    __syncthreads();
+    if (t < r1 - r0) {
+        const int row = r0 + t;
+        float total = 0.0f;
+        for (int c = chunk_ptr[row] - c0; c < chunk_ptr[row + 1] - c0; ++c) total = total + xs[c];
+        if (scale != 1.0f) total = total * scale;
+        float* yp = y + (int64_t)b * rows + row;
+        if (accumulate) total = *yp + total;
+        *yp = total;
+    }
+}
+
 }  // namespace
 
 // Same contract as flyverse/kernels/neural.cu:105 (launch_csr): p = [ptr int32, idx int32, values (float or half),
@@ -177,6 +257,31 @@ FH_EXPORT int launch_spmv_grouped(void** p, int rows, int cols, int ngroups, int
     hipLaunchKernelGGL(k_spmv_groups, dim3((unsigned)((ngroups + 3) / 4), (unsigned)batch), dim3(256), 0,
                        (hipStream_t)stream, (const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3],
                        (const float*)p[4], (float*)p[5], ngroups, rows, cols, scale, accumulate);
+    FH_LAUNCH_CHECK();
+}
+
+// LDS-staged product, same order as launch_csr, over the chunk-padded storage of launch_spmv_grouped:
+// p = [tile_row (ntiles+1): first row of each tile (at most 1024 chunks, at most 1024 rows), chunk_ptr (rows+1),
+// pidx (16*nchunks), pw (16*nchunks), x (B, cols), y (B, rows)]; range_cols = columns of x staged at a time
+// (0: as many as the LDS holds).  y = scale * W x (+ y if accumulate).
+FH_EXPORT int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batch, int range_cols, float scale,
+                              int accumulate, void* stream) {
+    if (rows < 0 || cols < 0 || batch < 0 || ntiles < 0 || batch > 65535 || range_cols < 0) return FH_ERR_ARGS;
+    if ((int64_t)rows * batch == 0 || ntiles == 0) return 0;
+    int dev = 0, lds = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return (int)e;
+    int rc = min(lds / 4, 16 * 1024);                  // a thread stages at most 16 values of a range
+    if (range_cols > 0) rc = min(rc, range_cols);
+    rc = max(rc, 1024);                                // the chunk sums reuse the buffer
+    if ((int64_t)rc * 4 > lds) return FH_ERR_SIZE;
+    e = hipFuncSetAttribute((const void*)k_spmv_lds, hipFuncAttributeMaxDynamicSharedMemorySize, rc * 4);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_spmv_lds, dim3((unsigned)ntiles, (unsigned)batch), dim3(1024), (size_t)rc * 4,
+                       (hipStream_t)stream, (const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3],
+                       (const float*)p[4], (float*)p[5], rows, cols, rc, scale, accumulate);
     FH_LAUNCH_CHECK();
 }
 

@@ -73,6 +73,22 @@ int launch_pull(void** p, int n, int batch, float a_s, int flags, void* stream);
 int launch_spmv_chunked(void** p, int rows, int cols, int nchunks, int batch, float scale, int accumulate,
                         void* stream);
 
+/* The launch_csr product in one pass over chunk-padded storage (row r's entries start at 16 * chunk_ptr[r], pads
+ * have idx -1 and add nothing): one wavefront per row group (consecutive rows of at most 64 chunks and 64 rows
+ * together, or one longer row) and fly; same additions in the same order as launch_csr.
+ * p = [group_row (ngroups+1), chunk_ptr (rows+1), pidx (16*nchunks), pw (16*nchunks), x (batch, cols),
+ * y (batch, rows)].  y = scale * W x (+ y if accumulate).  Replaces the _mv products of flyverse/optic.py:161-165. */
+int launch_spmv_grouped(void** p, int rows, int cols, int ngroups, int batch, float scale, int accumulate,
+                        void* stream);
+
+/* The launch_csr product with x staged in LDS, range of columns by range (a 4-byte gather through the vector cache
+ * moves a whole line from L2; from LDS it moves 4 bytes).  One 1024-thread workgroup per tile of at most 1024
+ * chunks and 1024 rows; same additions in the same order as launch_csr.  p = [tile_row (ntiles+1), chunk_ptr
+ * (rows+1), pidx (16*nchunks), pw (16*nchunks), x (batch, cols), y (batch, rows)]; range_cols = columns staged at
+ * a time (0: as many as the LDS holds).  Replaces the _mv products of flyverse/optic.py:161-165. */
+int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batch, int range_cols, float scale, int accumulate,
+                    void* stream);
+
 /* Photoreceptor low-pass, adaptation and contrast, one cell per photoreceptor (flyverse/optic.py:496-517).
  * p = [rad (batch, n_pr, 4), sens (n_pr, 4), I_lp, I_mean, fresh uint8 (batch), coefs(6), a_pr]; clears fresh. */
 int launch_photoreceptor(void** p, int n_pr, int batch, void* stream);

@@ -173,8 +173,10 @@ def test_pull_half_weights():
     assert_bits(N_(g), ref, "pull half")
 
 
-@pytest.mark.parametrize("B,kernel", [(1, "grouped"), (3, "grouped"), (8, "grouped"), (1, "chunked"), (3, "chunked")])
-def test_csr_products_bit_exact_and_close_to_float64(B, kernel):
+@pytest.mark.parametrize("B,kernel,range_cols", [(1, "grouped", 0), (3, "grouped", 0), (8, "grouped", 0),
+                                                  (1, "chunked", 0), (3, "chunked", 0), (1, "lds", 0),
+                                                  (3, "lds", 1024), (2, "lds", 1500)])
+def test_csr_products_bit_exact_and_close_to_float64(B, kernel, range_cols):
     rng = np.random.default_rng(5 + B)
     rows, cols = 2000, 1700
     M = random_csr(rng, rows, cols, 0.03, long_rows=(0, 999, 1998), long_len=1650)
@@ -188,7 +190,8 @@ def test_csr_products_bit_exact_and_close_to_float64(B, kernel):
     x = rng.standard_normal((B, cols)).astype(np.float32)
     x[:, ::3] = 0
     ref = oracle.spmv(M.indptr, M.indices, M.data, x)
-    csr = hip.CSR(M, DEV, kernel=kernel)
+    csr = hip.CSR(M, DEV, kernel=kernel, range_cols=range_cols)
+    assert csr.kernel == kernel
     y = csr.matvec(T(x))
     assert_bits(N_(y), ref, kernel)
     acc = T(rng.standard_normal((B, rows)).astype(np.float32))
