@@ -19,6 +19,8 @@
 // launch_packed_run runs a whole rollout in one cooperative launch with a grid barrier between windows.
 #include "common.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int kSlots = 64;
@@ -96,34 +98,9 @@ __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
         const float drive = a.drive[r], p = a.pois[r];
         const uint64_t seed = a.seeds[a.inst[r]];
         const uint32_t loc = (uint32_t)a.local[r];
-        for (int k = 0; k < K; ++k) {
+        // LIF update of step k (uniform across the wavefront: every lane computes the same row)
+        auto lif_step = [&](int k, float acc) {
             const int t = T0 + k;
-            const uint64_t* slot = a.ring + (int64_t)((t - a.delay) & (kSlots - 1)) * a.words;
-            float acc = g * a_s;
-            for (int base = e0; base < e1; base += 4 * kWave) {
-                int pre[4];
-                bool act[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int e = base + u * kWave + lane;
-                    pre[u] = e < e1 ? a.idx[e] : -1;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    act[u] = pre[u] >= 0 && ((slot[pre[u] >> 6] >> (pre[u] & 63)) & 1ull);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (base + u * kWave >= e1) break;
-                    const float c = act[u] ? a.w[base + u * kWave + lane] : 0.0f;
-                    uint64_t m = __builtin_amdgcn_ballot_w64(act[u]);
-                    while (m) {
-                        const int j = __builtin_ctzll(m);
-                        acc = acc + fh_readlane(c, j);
-                        m &= m - 1;
-                    }
-                }
-            }
-            // LIF update (uniform across the wavefront: every lane computes the same row)
             float target = v_rest + acc;
             target = target + drive;
             float dv = v - target;
@@ -143,6 +120,89 @@ __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
             if (fired && lane == 0)
                 atomicOr((unsigned long long*)(a.ring + (int64_t)(t & (kSlots - 1)) * a.words + (r >> 6)),
                          1ull << (r & 63));
+        };
+        auto slot_of = [&](int k) {
+            return a.ring + (int64_t)((T0 + k - a.delay) & (kSlots - 1)) * a.words;
+        };
+        // at most NC * 64 in-edges: presynaptic indices and weights stay in registers for the window, and the bit
+        // words of step k + 1 are gathered before step k is folded (two buffers alternating, no copies)
+        auto cached = [&](auto nc_tag) {
+            constexpr int NC = decltype(nc_tag)::value;
+            int pre[NC];
+            float wc[NC];
+            bool val[NC];
+            const int last = max(e1 - 1, e0);
+#pragma unroll
+            for (int u = 0; u < NC; ++u) {
+                const int e = e0 + u * kWave + lane;
+                val[u] = e < e1;
+                pre[u] = e1 > e0 ? a.idx[min(e, last)] : 0;
+                wc[u] = e1 > e0 ? a.w[min(e, last)] : 0.0f;
+            }
+            const int nch = (e1 - e0 + kWave - 1) / kWave;
+            uint64_t wa[NC], wb[NC];
+            auto look = [&](int k, uint64_t* wd) {
+                const uint64_t* slot = slot_of(k);
+#pragma unroll
+                for (int u = 0; u < NC; ++u) wd[u] = slot[pre[u] >> 6];
+            };
+            auto step = [&](int k, const uint64_t* wd) {
+                float acc = g * a_s;
+#pragma unroll
+                for (int u = 0; u < NC; ++u) {
+                    if (u < nch) {
+                        const bool act = val[u] && ((wd[u] >> (pre[u] & 63)) & 1ull);
+                        uint64_t m = __builtin_amdgcn_ballot_w64(act);
+                        while (m) {
+                            const int j = __builtin_ctzll(m);
+                            acc = acc + fh_readlane(wc[u], j);
+                            m &= m - 1;
+                        }
+                    }
+                }
+                lif_step(k, acc);
+            };
+            look(0, wa);
+            for (int k = 0; k < K; k += 2) {
+                look(min(k + 1, K - 1), wb);
+                step(k, wa);
+                look(min(k + 2, K - 1), wa);
+                if (k + 1 < K) step(k + 1, wb);
+            }
+        };
+        if (e1 - e0 <= 4 * kWave) {
+            cached(std::integral_constant<int, 4>{});
+        } else if (e1 - e0 <= 8 * kWave) {
+            cached(std::integral_constant<int, 8>{});
+        } else {
+            for (int k = 0; k < K; ++k) {
+                const uint64_t* slot = slot_of(k);
+                float acc = g * a_s;
+                for (int base = e0; base < e1; base += 4 * kWave) {
+                    int pre[4];
+                    bool act[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int e = base + u * kWave + lane;
+                        pre[u] = e < e1 ? a.idx[e] : -1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        act[u] = pre[u] >= 0 && ((slot[pre[u] >> 6] >> (pre[u] & 63)) & 1ull);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (base + u * kWave >= e1) break;
+                        const float c = act[u] ? a.w[base + u * kWave + lane] : 0.0f;
+                        uint64_t m = __builtin_amdgcn_ballot_w64(act[u]);
+                        while (m) {
+                            const int j = __builtin_ctzll(m);
+                            acc = acc + fh_readlane(c, j);
+                            m &= m - 1;
+                        }
+                    }
+                }
+                lif_step(k, acc);
+            }
         }
         if (lane == 0) { a.v[r] = v; a.g[r] = g; a.refrac[r] = ref; a.rate[r] = rate; a.counts[r] = cnt; }
     }

@@ -94,7 +94,8 @@ def tune_packed(pb, steps: int = 180, windows=None, grids=None, use_cache=True, 
         return table[key] | {"cached": True}
     cus = torch.cuda.get_device_properties(pb.device).multi_processor_count
     windows = windows or sorted({w for w in (pb.n_delay, (pb.n_delay + 1) // 2) if w >= 1})
-    grids = grids or sorted({min(pb.max_grid, cus * m) for m in (2, 4, 6)})
+    cap = pb.max_grid if pb.mode == "coop" else cus * 32
+    grids = grids or sorted({min(cap, cus * m) for m in (4, 8, 12, 16, 24)})
     trials = []
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for w, g in itertools.product(windows, grids):

@@ -86,7 +86,7 @@ def test_full_size_instances_and_launch_shapes():
         assert np.array_equal(got["v"], om.st.v[0]) and np.array_equal(got["counts"], om.st.spike_counts[0])
     ref = {name: x.clone() for name, x in (("v", pb.v), ("counts", pb.counts), ("rate", pb.rate))}
     assert ref["counts"].sum().item() > 0
-    for window, grid, mode in ((6, 256, "eager"), (9, pb.max_grid, "graph"), (18, 1000, "graph"), (18, 64, "coop")):
+    for window, grid, mode in ((6, 256, "eager"), (9, pb.max_grid, "graph"), (18, 4096, "graph"), (18, 64, "coop")):
         pb.configure(window=window, grid=grid, mode=mode)
         for _ in range(3):
             pb.run(40)
