@@ -29,7 +29,7 @@ def grads64(W0, ut, P, target, readout, tau, bias, gain):
     th = [torch.tensor(x, dtype=torch.float64, requires_grad=True) for x in (tau, bias, gain)]
     loss = loss64(*th, W0, ut, P, target, readout)
     loss.backward()
-    return float(loss), [x.grad.numpy() for x in th]
+    return loss.item(), [x.grad.numpy() for x in th]
 
 
 def test_float64_reference_matches_finite_differences():
