@@ -112,29 +112,36 @@ def time_lif_launches(bb):
 
 
 def spmv_roofline(conn, retina, op, reps=50):
-    """The optic recurrent product y = W_rr x (flyverse/optic.py:474-483) at B=1 on the full-scale graph (hip.CSR,
-    grouped kernel): mean duration over `reps` launches timed with HIP events on the launch stream, and algorithmic
-    bytes 8 nnz + 4 (rows + 1) + 4 (cols + rows)."""
+    """The optic recurrent product y = W_rr x (flyverse/optic.py:474-483) at B=1 on the full-scale graph, grouped
+    kernel (hip.CSR default): mean duration over `reps` launches timed with HIP events on the launch stream, and
+    algorithmic bytes 8 nnz + 4 (rows + 1) + 4 (cols + rows)."""
     import torch
     from flyhip import hip
     from flyhip.optic import optic_matrices
     M = optic_matrices(conn, retina, op)["M_rr"]
-    csr = hip.CSR(M, "cuda")
-    x = torch.rand(1, M.shape[1], device="cuda")
-    y = csr.matvec(x)
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(reps):
-        csr.matvec(x, out=y)
-    e1.record(stream)
-    e1.synchronize()
-    us = e0.elapsed_time(e1) * 1000 / reps
-    alg = 8 * M.nnz + 4 * (M.shape[0] + 1) + 4 * (M.shape[0] + M.shape[1])
-    gbs = alg / (us * 1e-6) / 1e9
+    out = {}
+    for kernel, rc in (("lds", 0), ("grouped", 0), ("chunked", 0)):
+        csr = hip.CSR(M, "cuda", kernel=kernel, range_cols=rc)
+        kernel = kernel + (f"{rc}" if rc else "")
+        x = torch.rand(1, M.shape[1], device="cuda")
+        y = csr.matvec(x)
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            csr.matvec(x, out=y)
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1000 / reps
+        alg = 8 * M.nnz + 4 * (M.shape[0] + 1) + 4 * (M.shape[0] + M.shape[1])
+        out[kernel] = (us, alg / (us * 1e-6) / 1e9)
+        del csr
+    best = min(out, key=lambda k: out[k][0])
+    us, gbs = out[best]
     return {"spmv_wrr_b1_us": round(us, 2), "spmv_wrr_b1_gbs": round(gbs, 1),
-            "spmv_wrr_b1_frac": round(gbs / HBM_PEAK_GBS, 4), "spmv_wrr_nnz": int(M.nnz)}
+            "spmv_wrr_b1_frac": round(gbs / HBM_PEAK_GBS, 4), "spmv_wrr_nnz": int(M.nnz), "spmv_wrr_kernel": best,
+            **{f"spmv_wrr_b1_{k}_us": round(v[0], 2) for k, v in out.items()}}
 
 
 def measured_traffic(shape):
@@ -338,8 +345,9 @@ def main():
         return
 
     spmv = spmv_roofline(conn, retina, op)
-    log(f"W_rr product, B=1: {spmv['spmv_wrr_b1_us']} us, {spmv['spmv_wrr_b1_gbs']} GB/s algorithmic "
-        f"({spmv['spmv_wrr_b1_frac'] * 100:.1f} % of peak)")
+    log(f"W_rr product, B=1: {spmv['spmv_wrr_b1_us']} us ({spmv['spmv_wrr_kernel']}), {spmv['spmv_wrr_b1_gbs']} GB/s "
+        f"algorithmic ({spmv['spmv_wrr_b1_frac'] * 100:.1f} % of peak); all: "
+        f"{ {k: v for k, v in spmv.items() if k.endswith('_us') and k != 'spmv_wrr_b1_us'} }")
 
     # ---------------------------------------------------------------- B = 1, plain kernels
     log("B=1 plain path: setup")

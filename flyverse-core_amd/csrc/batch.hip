@@ -438,15 +438,12 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
 }
 
 // ------------------------------------------------------------------ optic lobe, fly-minor
-// Products over a (cols, 64) input with segment-padded storage: row r owns chunks [chunk_ptr[r], chunk_ptr[r+1]),
-// chunk c is entries [16 c, 16 c + 16) of pidx / pw, every (row, column block) segment is padded to whole chunks,
-// pads have idx -1 and are skipped.  A wavefront
+// Chunked products over a (cols, 64) input with chunk-padded storage: row r owns chunks [chunk_ptr[r],
+// chunk_ptr[r+1]), chunk c is entries [16 c, 16 c + 16) of pidx / pw, pads have idx -1 and are skipped.  A wavefront
 // walks the chunks of its rows (sched order, stride = number of wavefronts) as one stream: the scalar loads of a
 // chunk's indices and weights and its 16 row gathers (256 bytes each, one per entry) are issued one chunk ahead, so
-// the fold of chunk c overlaps the loads of chunk c + 1.  Each chunk is folded from +0 in entry order, the chunk sums
-// of a segment into its partial, the partials of a row into its total (csrc/spmv.hip order: columns ascend within a
-// row, so a chunk opens a new segment exactly when its first column reaches the current block's end);
-// finish(row, total) consumes the row total.
+// the fold of chunk c overlaps the loads of chunk c + 1.  Each chunk is folded from +0 in entry order and the chunk
+// sums of a row in chunk order (csrc/spmv.hip order); finish(row, total) consumes the row total.
 struct FmBuf {
     uint32_t idx[16];
     float w[16];
@@ -458,9 +455,8 @@ struct FmCursor { int si, r, c, ce; };
 template <class Finish>
 __device__ __forceinline__ void fm_chunk_stream(const int* __restrict__ chunk_ptr, cu32p pidx, cf32p pw,
                                                 const int* __restrict__ sched, const float* __restrict__ x, int rows,
-                                                int cols, int lane, Finish finish) {
+                                                int lane, Finish finish) {
     const int nw = gridDim.x * 4;
-    const int width = max(1, (cols + 7) / 8);          // column block width (8 blocks)
     // move to the first non-empty row at or after schedule index si (empty rows are finished with a zero total)
     auto seek = [&](FmCursor& q, int si) {
         q.si = si;
@@ -485,27 +481,14 @@ __device__ __forceinline__ void fm_chunk_stream(const int* __restrict__ chunk_pt
 #pragma unroll
         for (int j = 0; j < 16; ++j) b.x[j] = x[(size_t)max((int)b.idx[j], 0) * 64 + lane];
     };
-    float total = 0.0f, part = 0.0f;
-    int bend = 0;                                       // end column of the open segment's block (0: none open)
+    float total = 0.0f;
     auto fold = [&](const FmCursor& q, const FmBuf& b) {
         float s = 0.0f;
 #pragma unroll
         for (int j = 0; j < 16; ++j)
             if ((int)b.idx[j] >= 0) { const float p = b.w[j] * b.x[j]; s = s + p; }
-        const int first = (int)b.idx[0];                // a chunk's first entry is never a pad
-        if (first >= bend) {
-            if (bend > 0) total = total + part;
-            part = 0.0f;
-            bend = (first / width + 1) * width;
-        }
-        part = part + s;
-        if (q.c + 1 == q.ce) {
-            total = total + part;
-            finish(q.r, total);
-            total = 0.0f;
-            part = 0.0f;
-            bend = 0;
-        }
+        total = total + s;
+        if (q.c + 1 == q.ce) { finish(q.r, total); total = 0.0f; }
     };
     FmCursor p0;
     seek(p0, blockIdx.x * 4 + wave_id());
@@ -531,9 +514,9 @@ __device__ __forceinline__ void fm_chunk_stream(const int* __restrict__ chunk_pt
 __global__ __launch_bounds__(256) void k_spmm_fm(const int* __restrict__ chunk_ptr, const int* __restrict__ pidx,
                                                  const float* __restrict__ pw, const int* __restrict__ sched,
                                                  const float* __restrict__ x, float* __restrict__ y, int rows,
-                                                 int cols, float scale) {
+                                                 float scale) {
     const int lane = threadIdx.x & 63;
-    fm_chunk_stream(chunk_ptr, (cu32p)pidx, (cf32p)pw, sched, x, rows, cols, lane, [&](int r, float total) {
+    fm_chunk_stream(chunk_ptr, (cu32p)pidx, (cf32p)pw, sched, x, rows, lane, [&](int r, float total) {
         if (scale != 1.0f) total = total * scale;
         y[(int64_t)r * 64 + lane] = total;
     });
@@ -551,7 +534,7 @@ __global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict_
                                                           float ga, float aad) {
     const int lane = threadIdx.x & 63;
     const bool live = (*live_mask >> lane) & 1ull;
-    fm_chunk_stream(chunk_ptr, (cu32p)pidx, (cf32p)pw, sched, dr_cur, rows, rows, lane, [&](int r, float total) {
+    fm_chunk_stream(chunk_ptr, (cu32p)pidx, (cf32p)pw, sched, dr_cur, rows, lane, [&](int r, float total) {
         const int64_t q = (int64_t)r * 64 + lane;
         const float old = dr_cur[q];
         if (!live) { dr_next[q] = old; return; }
@@ -723,11 +706,11 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
 }
 
 // y (rows, 64) = scale * W x (cols, 64).  p = [chunk_ptr (rows+1), pidx (16 per chunk, pad -1), pw, sched, x, y].
-FH_EXPORT int launch_spmm_fm(void** p, int rows, int cols, float scale, int grid, void* stream) {
-    if (rows < 0 || cols < 0 || grid <= 0) return FH_ERR_ARGS;
+FH_EXPORT int launch_spmm_fm(void** p, int rows, float scale, int grid, void* stream) {
+    if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
     if (!rows) return 0;
     hipLaunchKernelGGL(k_spmm_fm, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const int*)p[0], (const int*)p[1],
-                       (const float*)p[2], (const int*)p[3], (const float*)p[4], (float*)p[5], rows, cols, scale);
+                       (const float*)p[2], (const int*)p[3], (const float*)p[4], (float*)p[5], rows, scale);
     FH_LAUNCH_CHECK();
 }
 

@@ -1,25 +1,17 @@
 // CSR products for the optic lobe (flyverse/optic.py:161-165; the warp kernel flyverse/kernels/neural.cu:91-112).
 //
-// Summation order (defined by this build, reproduced bit for bit by the CPU oracle): the columns are cut into
-// kBlocks blocks of width ceil(cols / kBlocks); a row's stored entries (ascending column) in one block form a
-// segment; a segment is cut into chunks of kChunk consecutive entries, each chunk a sequential fp32 fold of its
-// products from +0 (each product rounded first); the segment partial is a sequential fold of its chunk sums from +0;
-// the row total is a sequential fold of the partials of its non-empty segments from +0, in block order.  The blocks
-// let a product be split by column block (each block's slice of x kept in one cache) with the partials combined in
-// order afterwards.  The reference's CPU order (torch sparse mm) is not pinned by anything it ships; agreement with
-// it is to 1e-5 relative.
+// Summation order (defined by this build, reproduced bit for bit by the CPU oracle): a row's stored entries are cut
+// into chunks of kChunk consecutive entries; each chunk is a sequential fp32 fold of its products from +0 (each
+// product rounded first), and the row total is a sequential fold of the chunk sums from +0.  The reference's CPU
+// order (torch sparse mm) is not pinned by anything it ships; agreement with it is to 1e-5 relative.
 #include "common.h"
 
 namespace {
 
 constexpr int kChunk = 16;
-constexpr int kBlocks = 8;
 
-__device__ __forceinline__ int block_width(int cols) { return max(1, (cols + kBlocks - 1) / kBlocks); }
-
-// One wavefront per (fly, row) over the (B, cols) fly-major x: the row's segments in order; within a segment lane k
-// folds chunk k of each round of 64 chunks, then the wave folds the chunk sums in order.  With half weights x is
-// rounded to half first (neural.cu:99).
+// One wavefront per (fly, row) over the (B, cols) fly-major x: lane k folds chunk k of each round of 64 chunks,
+// then the wave folds the chunk sums in order.  With half weights x is rounded to half first (neural.cu:99).
 template <typename W>
 __global__ __launch_bounds__(256) void k_csr(const int* __restrict__ ptr, const int* __restrict__ idx,
                                              const W* __restrict__ w, const float* __restrict__ x,
@@ -29,71 +21,85 @@ __global__ __launch_bounds__(256) void k_csr(const int* __restrict__ ptr, const 
     int lane = threadIdx.x & 63;
     int b = (int)(gw / rows), row = (int)(gw % rows);
     const float* xb = x + (int64_t)b * cols;
-    const int width = block_width(cols);
-    const int start = ptr[row], end = ptr[row + 1];
+    int start = ptr[row], end = ptr[row + 1];
     float total = 0.0f;
-    int e = start;
-    while (e < end) {
-        // the segment [e, se): entries whose column lies in the block of idx[e] (columns ascend, so a prefix)
-        const int bend = (idx[e] / width + 1) * width;
-        int se = e;
-        for (;;) {
-            const int k = se + lane;
-            const uint64_t bal = __builtin_amdgcn_ballot_w64(k < end && idx[k] < bend);
-            const int n = __builtin_popcountll(bal);
-            se += n;
-            if (n < kWave) break;
+    for (int cb = start; cb < end; cb += kWave * kChunk) {
+        int cs = cb + lane * kChunk, ce = min(cs + kChunk, end);
+        float s = 0.0f;
+        for (int e = cs; e < ce; ++e) {
+            float xv = xb[idx[e]];
+            if (sizeof(W) == 2) xv = __half2float(__float2half_rn(xv));
+            float c = (float)w[e] * xv;
+            s = s + c;
         }
-        float part = 0.0f;
-        for (int cb = e; cb < se; cb += kWave * kChunk) {
-            const int cs = cb + lane * kChunk, ce = min(cs + kChunk, se);
-            float s = 0.0f;
-            for (int q = cs; q < ce; ++q) {
-                float xv = xb[idx[q]];
-                if (sizeof(W) == 2) xv = __half2float(__float2half_rn(xv));
-                float c = (float)w[q] * xv;
-                s = s + c;
-            }
-            const int nch = min(kWave, (se - cb + kChunk - 1) / kChunk);
-            for (int j = 0; j < nch; ++j) part = part + fh_readlane(s, j);
-        }
-        total = total + part;
-        e = se;
+        int nch = min(kWave, (end - cb + kChunk - 1) / kChunk);
+        for (int j = 0; j < nch; ++j) total = total + fh_readlane(s, j);
     }
     if (lane == 0) y[gw] = total;
 }
 
-// Single-pass grouped product over segment-padded storage: every segment's entries are stored padded to whole chunks
+// Binned variant for the (B, cols) layout: one lane per chunk over a precomputed chunk table, then one lane per row
+// folds its chunk sums.  chunk_row is not needed: chunk c of the matrix covers entries [c0[r] + k*kChunk, ...) of
+// row r where c0 = chunk_ptr.  Pass 1 writes partial[b, c]; pass 2 folds them per row in chunk order.
+__global__ __launch_bounds__(256) void k_chunk_sums(const int* __restrict__ ptr, const int* __restrict__ chunk_row,
+                                                    const int* __restrict__ chunk_ptr, const int* __restrict__ idx,
+                                                    const float* __restrict__ w, const float* __restrict__ x,
+                                                    float* __restrict__ partial, int nchunks, int cols, int batch) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nchunks * batch) return;
+    int b = (int)(t / nchunks), c = (int)(t % nchunks);
+    int r = chunk_row[c];
+    int cs = ptr[r] + (c - chunk_ptr[r]) * kChunk, ce = min(cs + kChunk, ptr[r + 1]);
+    const float* xb = x + (int64_t)b * cols;
+    float s = 0.0f;
+#pragma unroll 4
+    for (int e = cs; e < ce; ++e) {
+        float cc = w[e] * xb[idx[e]];
+        s = s + cc;
+    }
+    partial[t] = s;
+}
+
+__global__ __launch_bounds__(256) void k_chunk_fold(const int* __restrict__ chunk_ptr, const float* __restrict__ partial,
+                                                    float* __restrict__ y, int rows, int nchunks, int batch,
+                                                    float scale, int accumulate) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)rows * batch) return;
+    int b = (int)(t / rows), r = (int)(t % rows);
+    const float* pb = partial + (int64_t)b * nchunks;
+    float total = 0.0f;
+    for (int c = chunk_ptr[r]; c < chunk_ptr[r + 1]; ++c) total = total + pb[c];
+    if (scale != 1.0f) total = total * scale;
+    if (accumulate) total = y[t] + total;
+    y[t] = total;
+}
+
+// Single-pass grouped product over chunk-padded storage: every row's entries are stored padded to whole chunks
 // (pad entries have idx -1 and are skipped, so they add nothing, not even 0 * x), chunk c of the matrix occupying
-// entries [16 c, 16 c + 16); a chunk's block is that of its first entry.  Consecutive rows are grouped on the host so
-// a group has at most 64 chunks and 64 rows; one wavefront per (group, fly): lane j forms the sum of the group's
-// chunk j from two 64-byte vector loads, the sums and blocks go through LDS, and lane r folds row r's chunk sums into
-// segment partials and the partials into the row total, in order.  A row of more than 64 chunks is a group of its
-// own and its wavefront folds the chunks of each round of 64 in order.
+// entries [16 c, 16 c + 16).  Consecutive rows are grouped on the host so a group has at most 64 chunks and 64 rows;
+// one wavefront per (group, fly): lane j forms the sum of the group's chunk j from two 64-byte vector loads, the sums
+// go through LDS, and lane r folds row r's chunk sums in order.  A row of more than 64 chunks is a group of its own
+// and its wavefront folds the chunk sums of each round of 64 in order.  Same additions, same order as k_csr.
 __global__ __launch_bounds__(256) void k_spmv_groups(const int* __restrict__ group_row, const int* __restrict__ chunk_ptr,
                                                      const int4* __restrict__ pidx, const float4* __restrict__ pw,
                                                      const float* __restrict__ x, float* __restrict__ y, int ngroups,
                                                      int rows, int cols, float scale, int accumulate) {
     __shared__ float sums[4][kWave];
-    __shared__ int blks[4][kWave];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = blockIdx.x * 4 + wv;
     if (g >= ngroups) return;
     const int b = blockIdx.y;
     const float* xb = x + (int64_t)b * cols;
-    const int width = block_width(cols);
     const int r0 = group_row[g], r1 = group_row[g + 1];
     const int c0 = chunk_ptr[r0], c1 = chunk_ptr[r1];
-    // the sum of chunk c and its block
-    auto chunk_sum = [&](int c, int& blk) {
+    auto chunk_sum = [&](int c) {
         const int4* ip = pidx + (int64_t)c * 4;
         const float4* wp = pw + (int64_t)c * 4;
         int4 iv[4];
         float4 wv4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) { iv[q] = ip[q]; wv4[q] = wp[q]; }
-        blk = iv[0].x / width;
         float xv[16];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -120,48 +126,105 @@ __global__ __launch_bounds__(256) void k_spmv_groups(const int* __restrict__ gro
         *yp = total;
     };
     if (c1 - c0 <= kWave) {
-        int blk = -1;
-        const float s = lane < c1 - c0 ? chunk_sum(c0 + lane, blk) : 0.0f;
+        const float s = lane < c1 - c0 ? chunk_sum(c0 + lane) : 0.0f;
         sums[wv][lane] = s;
-        blks[wv][lane] = blk;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lane < r1 - r0) {
             const int row = r0 + lane;
-            float total = 0.0f, part = 0.0f;
-            int prev = -1;
-            for (int c = chunk_ptr[row] - c0; c < chunk_ptr[row + 1] - c0; ++c) {
-                const int bc = blks[wv][c];
-                if (bc != prev) {
-                    if (prev >= 0) total = total + part;
-                    part = 0.0f;
-                    prev = bc;
-                }
-                part = part + sums[wv][c];
-            }
-            if (prev >= 0) total = total + part;
+            float total = 0.0f;
+            for (int c = chunk_ptr[row] - c0; c < chunk_ptr[row + 1] - c0; ++c) total = total + sums[wv][c];
             finish(total, row);
         }
     } else {                                       // one long row
-        float total = 0.0f, part = 0.0f;
-        int prev = -1;
+        float total = 0.0f;
         for (int cb = c0; cb < c1; cb += kWave) {
-            int blk = -1;
-            const float s = cb + lane < c1 ? chunk_sum(cb + lane, blk) : 0.0f;
+            const float s = cb + lane < c1 ? chunk_sum(cb + lane) : 0.0f;
             const int n = min(kWave, c1 - cb);
-            for (int j = 0; j < n; ++j) {
-                const int bc = __builtin_amdgcn_readlane(blk, j);
-                if (bc != prev) {
-                    if (prev >= 0) total = total + part;
-                    part = 0.0f;
-                    prev = bc;
-                }
-                part = part + fh_readlane(s, j);
+            for (int j = 0; j < n; ++j) total = total + fh_readlane(s, j);
+        }
+        if (lane == 0) finish(total, r0);
+    }
+}
+
+// LDS-staged product over chunk-padded storage: the gathers of x go to LDS instead of the vector caches (a 4-byte
+// gather through the vector L1 moves a whole cache line from L2).  One workgroup of 1024 threads per tile of at
+// most 1024 consecutive chunks (whole rows, host-built); thread i holds chunk i of the tile in registers.  x is
+// staged range by range (each range as many columns as the LDS buffer holds); within a range a thread adds, in
+// entry order, the entries of its chunk whose column lies in the range, so over the ranges every entry is added
+// once, in column order, which is the chunk's entry order.  Then thread i folds the chunk sums of the tile's row i
+// in chunk order.  Same additions, same order as k_csr.
+__global__ __launch_bounds__(1024) void k_spmv_lds(const int* __restrict__ tile_row, const int* __restrict__ chunk_ptr,
+                                                  const int4* __restrict__ pidx, const float4* __restrict__ pw,
+                                                  const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                  int cols, int range_cols, float scale, int accumulate) {
+    extern __shared__ float xs[];                  // range_cols floats (reused for the chunk sums)
+    const int b = blockIdx.y;
+    const float* xb = x + (int64_t)b * cols;
+    const int r0 = tile_row[blockIdx.x], r1 = tile_row[blockIdx.x + 1];
+    const int c0 = chunk_ptr[r0], c1 = chunk_ptr[r1];
+    const int t = threadIdx.x;
+    const bool mine = c0 + t < c1;
+    int iv[16];
+    float wv[16];
+    if (mine) {
+        const int4* ip = pidx + (int64_t)(c0 + t) * 4;
+        const float4* wp = pw + (int64_t)(c0 + t) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 i4 = ip[q];
+            const float4 w4 = wp[q];
+            iv[4 * q] = i4.x; iv[4 * q + 1] = i4.y; iv[4 * q + 2] = i4.z; iv[4 * q + 3] = i4.w;
+            wv[4 * q] = w4.x; wv[4 * q + 1] = w4.y; wv[4 * q + 2] = w4.z; wv[4 * q + 3] = w4.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) { iv[j] = -1; wv[j] = 0.0f; }
+    }
+    // the next range's values are loaded into registers while the current range is folded (at most 16 per thread)
+    float xr[16];
+    auto load_range = [&](int lo) {
+        const int n = min(range_cols, cols - lo);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = q * 1024 + t;
+            xr[q] = k < n ? xb[lo + k] : 0.0f;
+        }
+    };
+    float s = 0.0f;
+    load_range(0);
+    for (int lo = 0; lo < cols; lo += range_cols) {
+        const int n = min(range_cols, cols - lo);
+        __syncthreads();                           // the previous range is no longer read
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = q * 1024 + t;
+            if (k < n) xs[k] = xr[q];
+        }
+        __syncthreads();
+        if (lo + range_cols < cols) load_range(lo + range_cols);
+        const unsigned un = (unsigned)n;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const unsigned off = (unsigned)(iv[j] - lo);   // in range iff off < n (pads: idx -1 never)
+            if (iv[j] >= 0 && off < un) {
+                const float c = wv[j] * xs[off];
+                s = s + c;
             }
         }
-        if (prev >= 0) total = total + part;
-        if (lane == 0) finish(total, r0);
+    }
+    __syncthreads();
+    xs[t] = s;
+    __syncthreads();
+    if (t < r1 - r0) {
+        const int row = r0 + t;
+        float total = 0.0f;
+        for (int c = chunk_ptr[row] - c0; c < chunk_ptr[row + 1] - c0; ++c) total = total + xs[c];
+        if (scale != 1.0f) total = total * scale;
+        float* yp = y + (int64_t)b * rows + row;
+        if (accumulate) total = *yp + total;
+        *yp = total;
     }
 }
 
@@ -183,7 +246,7 @@ FH_EXPORT int launch_csr(void** p, int rows, int cols, int batch, int half, void
     FH_LAUNCH_CHECK();
 }
 
-// Single-pass grouped product, same order as launch_csr, over segment-padded storage: p = [group_row (ngroups+1):
+// Single-pass grouped product, same order as launch_csr, over chunk-padded storage: p = [group_row (ngroups+1):
 // first row of each group, chunk_ptr (rows+1), pidx int32 (16 * nchunks, pad = -1), pw float (16 * nchunks),
 // x (B, cols), y (B, rows)].  y = scale * (W x) (+ y if accumulate).  Groups hold at most 64 chunks and 64 rows,
 // or one row of more than 64 chunks.
@@ -194,5 +257,46 @@ FH_EXPORT int launch_spmv_grouped(void** p, int rows, int cols, int ngroups, int
     hipLaunchKernelGGL(k_spmv_groups, dim3((unsigned)((ngroups + 3) / 4), (unsigned)batch), dim3(256), 0,
                        (hipStream_t)stream, (const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3],
                        (const float*)p[4], (float*)p[5], ngroups, rows, cols, scale, accumulate);
+    FH_LAUNCH_CHECK();
+}
+
+// LDS-staged product, same order as launch_csr, over the chunk-padded storage of launch_spmv_grouped:
+// p = [tile_row (ntiles+1): first row of each tile (at most 1024 chunks, at most 1024 rows), chunk_ptr (rows+1),
+// pidx (16*nchunks), pw (16*nchunks), x (B, cols), y (B, rows)]; range_cols = columns of x staged at a time
+// (0: as many as the LDS holds).  y = scale * W x (+ y if accumulate).
+FH_EXPORT int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batch, int range_cols, float scale,
+                              int accumulate, void* stream) {
+    if (rows < 0 || cols < 0 || batch < 0 || ntiles < 0 || batch > 65535 || range_cols < 0) return FH_ERR_ARGS;
+    if ((int64_t)rows * batch == 0 || ntiles == 0) return 0;
+    int dev = 0, lds = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return (int)e;
+    int rc = min(lds / 4, 16 * 1024);                  // a thread stages at most 16 values of a range
+    if (range_cols > 0) rc = min(rc, range_cols);
+    rc = max(rc, 1024);                                // the chunk sums reuse the buffer
+    if ((int64_t)rc * 4 > lds) return FH_ERR_SIZE;
+    e = hipFuncSetAttribute((const void*)k_spmv_lds, hipFuncAttributeMaxDynamicSharedMemorySize, rc * 4);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_spmv_lds, dim3((unsigned)ntiles, (unsigned)batch), dim3(1024), (size_t)rc * 4,
+                       (hipStream_t)stream, (const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3],
+                       (const float*)p[4], (float*)p[5], rows, cols, rc, scale, accumulate);
+    FH_LAUNCH_CHECK();
+}
+
+// Chunk-binned product, same order as launch_csr: p = [ptr, chunk_row (nchunks), chunk_ptr (rows+1), idx, w,
+// x (B, cols), y (B, rows), partial (B, nchunks) scratch].  y = scale * (W x) (+ y if accumulate).
+FH_EXPORT int launch_spmv_chunked(void** p, int rows, int cols, int nchunks, int batch, float scale, int accumulate,
+                                  void* stream) {
+    if (rows < 0 || cols < 0 || batch < 0 || nchunks < 0) return FH_ERR_ARGS;
+    if ((int64_t)rows * batch == 0) return 0;
+    int64_t n1 = (int64_t)nchunks * batch, n2 = (int64_t)rows * batch;
+    if (n1)
+        hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                           (const int*)p[0], (const int*)p[1], (const int*)p[2], (const int*)p[3], (const float*)p[4],
+                           (const float*)p[5], (float*)p[7], nchunks, cols, batch);
+    hipLaunchKernelGGL(k_chunk_fold, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const int*)p[2], (const float*)p[7], (float*)p[6], rows, nchunks, batch, scale, accumulate);
     FH_LAUNCH_CHECK();
 }

@@ -32,8 +32,7 @@ int launch_optic_dr(void** p, int n, int batch, void* stream);
  * p = [v, adapt, dr, y, pr, spk, a(n), b(n)]. */
 int launch_optic(void** p, int n, int batch, float gain_rr, float adapt_gain, float a_ad, void* stream);
 
-/* y = W x for a row-major CSR, deterministic summation order (8 column blocks; 16-entry chunks within a block's
- * segment of the row; chunk sums, then block partials, folded in order).  Replaces launch_csr, neural.cu:105
+/* y = W x for a row-major CSR, deterministic chunked summation order.  Replaces launch_csr, neural.cu:105
  * (cuda.py:131-153).  p = [ptr(rows+1), idx, values (float, or half if half), x (batch, cols), y (batch, rows)]. */
 int launch_csr(void** p, int rows, int cols, int batch, int half, void* stream);
 
@@ -67,13 +66,28 @@ int launch_scale(float* x, int64_t count, float a, void* stream);
  * p = [ptr(n+1), idx, w (float, or half if flags & 1), x (batch, n), g (batch, n)], post-major CSR. */
 int launch_pull(void** p, int n, int batch, float a_s, int flags, void* stream);
 
-/* The launch_csr product in one pass over segment-padded storage (row r's chunks start at chunk_ptr[r]; every
- * (row, column block) segment is padded to whole 16-entry chunks; pads have idx -1 and add nothing): one wavefront per row group (consecutive rows of at most 64 chunks and 64 rows
+/* The launch_csr product through a precomputed chunk table (one lane per 16-entry chunk, then one lane per row),
+ * y = scale * W x (+ y if accumulate).  Replaces the _mv products of flyverse/optic.py:161-165, 579-580, 602.
+ * p = [ptr, chunk_row(nchunks), chunk_ptr(rows+1), idx, w, x (batch, cols), y (batch, rows), partial (batch,
+ * nchunks) scratch]. */
+int launch_spmv_chunked(void** p, int rows, int cols, int nchunks, int batch, float scale, int accumulate,
+                        void* stream);
+
+/* The launch_csr product in one pass over chunk-padded storage (row r's entries start at 16 * chunk_ptr[r], pads
+ * have idx -1 and add nothing): one wavefront per row group (consecutive rows of at most 64 chunks and 64 rows
  * together, or one longer row) and fly; same additions in the same order as launch_csr.
  * p = [group_row (ngroups+1), chunk_ptr (rows+1), pidx (16*nchunks), pw (16*nchunks), x (batch, cols),
  * y (batch, rows)].  y = scale * W x (+ y if accumulate).  Replaces the _mv products of flyverse/optic.py:161-165. */
 int launch_spmv_grouped(void** p, int rows, int cols, int ngroups, int batch, float scale, int accumulate,
                         void* stream);
+
+/* The launch_csr product with x staged in LDS, range of columns by range (a 4-byte gather through the vector cache
+ * moves a whole line from L2; from LDS it moves 4 bytes).  One 1024-thread workgroup per tile of at most 1024
+ * chunks and 1024 rows; same additions in the same order as launch_csr.  p = [tile_row (ntiles+1), chunk_ptr
+ * (rows+1), pidx (16*nchunks), pw (16*nchunks), x (batch, cols), y (batch, rows)]; range_cols = columns staged at
+ * a time (0: as many as the LDS holds).  Replaces the _mv products of flyverse/optic.py:161-165. */
+int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batch, int range_cols, float scale, int accumulate,
+                    void* stream);
 
 /* Photoreceptor low-pass, adaptation and contrast, one cell per photoreceptor (flyverse/optic.py:496-517).
  * p = [rad (batch, n_pr, 4), sens (n_pr, 4), I_lp, I_mean, fresh uint8 (batch), coefs(6), a_pr]; clears fresh. */
@@ -106,10 +120,9 @@ int launch_pull_bits(void** p, int n, int n_full, int batch, float a_s, int flag
  * dims = [n_rows, lanes, ring_slots, delay, record, grid, fold mode 0|1|2, row_major ring]. */
 int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream);
 
-/* y (rows, 64) = scale * W x (cols, 64), summation order of launch_csr.  p = [chunk_ptr (rows+1), pidx, pw (16 per
- * chunk, (row, column block) segments padded to whole chunks, pad idx -1), sched, x, y].
+/* y (rows, 64) = scale * W x (cols, 64), chunked order of launch_csr.  p = [ptr, idx, w, sched, x, y].
  * Replaces the per-frame optic products W_rp@a_pr, W_rs@s, W_sr@dr (flyverse/optic.py:579-580, 602). */
-int launch_spmm_fm(void** p, int rows, int cols, float scale, int grid, void* stream);
+int launch_spmm_fm(void** p, int rows, float scale, int grid, void* stream);
 
 /* One optic substep for 64 flies: y = W_rr dr_cur fused with the update of launch_optic; writes dr_next.
  * p = [ptr, idx, w, sched, dr_cur, dr_next, v, adapt, pr, spk, a, b, live_mask].  Replaces OpticLobe._substep

@@ -147,42 +147,29 @@ void orc_events(int n, int batch, const int64_t* out_ptr, const int32_t* out_pos
 
 /* ------------------------------------------------------------------ CSR product (A8)
  * y[b, r] = sum_j W[r, j] x[b, j] for the optic products (flyverse/optic.py:161-165, 474-483).  The reference's CPU
- * order (torch sparse mm) is not pinned by any reference vector, so this build DEFINES the order.  The columns are
- * cut into ORC_BLOCKS blocks of width ceil(cols / ORC_BLOCKS).  A row's stored entries (ascending column) whose
- * columns lie in one block form a segment; a segment is cut into chunks of 16 consecutive entries, each chunk a
- * sequential fp32 fold of its products from +0 (each product rounded first); the segment partial is a sequential
- * fold of its chunk sums from +0; the row total is a sequential fold of the partials of its non-empty segments from
- * +0, in block order.  The GPU kernels reproduce this order bit for bit; parity with the reference itself is to
- * 1e-5 relative (checked against float64 scipy in tests).  half != 0: weights are float16 values (given here as
- * float) and x is rounded to half before the product (neural.cu:99). */
+ * order (torch sparse mm) is not pinned by any reference vector, so this build DEFINES the order: the row's stored
+ * entries (ascending column) are cut into chunks of 16 consecutive entries; each chunk is a sequential fp32 fold of
+ * its products from +0 (each product rounded first); the row total is a sequential fold of the chunk sums from +0.
+ * The GPU kernels reproduce this order bit for bit; parity with the reference itself is to 1e-5 relative
+ * (checked against float64 scipy in tests).  half != 0: weights are float16 values (given here as float) and x is
+ * rounded to half before the product (neural.cu:99). */
 #define ORC_CHUNK 16
-#define ORC_BLOCKS 8
 static float orc_half_round(float x);
 void orc_spmv(int rows, int cols, int batch, const int64_t* ptr, const int32_t* idx, const float* w,
               const float* x, float* y, int half) {
-    const int64_t width = (cols + ORC_BLOCKS - 1) / ORC_BLOCKS;
     for (int b = 0; b < batch; ++b)
         for (int r = 0; r < rows; ++r) {
             float total = 0.0f;
-            int64_t e = ptr[r];
-            while (e < ptr[r + 1]) {
-                const int64_t block_end = (idx[e] / width + 1) * width;    /* first column past this segment */
-                int64_t se = e;
-                while (se < ptr[r + 1] && idx[se] < block_end) ++se;
-                float part = 0.0f;
-                for (int64_t cs = e; cs < se; cs += ORC_CHUNK) {
-                    int64_t ce = cs + ORC_CHUNK < se ? cs + ORC_CHUNK : se;
-                    float s = 0.0f;
-                    for (int64_t k = cs; k < ce; ++k) {
-                        float xv = x[(int64_t)b * cols + idx[k]];
-                        if (half) xv = orc_half_round(xv);
-                        float c = w[k] * xv;
-                        s = s + c;
-                    }
-                    part = part + s;
+            for (int64_t cs = ptr[r]; cs < ptr[r + 1]; cs += ORC_CHUNK) {
+                int64_t ce = cs + ORC_CHUNK < ptr[r + 1] ? cs + ORC_CHUNK : ptr[r + 1];
+                float s = 0.0f;
+                for (int64_t e = cs; e < ce; ++e) {
+                    float xv = x[(int64_t)b * cols + idx[e]];
+                    if (half) xv = orc_half_round(xv);
+                    float c = w[e] * xv;
+                    s = s + c;
                 }
-                total = total + part;
-                e = se;
+                total = total + s;
             }
             y[(int64_t)b * rows + r] = total;
         }

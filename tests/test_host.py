@@ -141,41 +141,3 @@ def test_gpu_entry_points_refuse_the_cpu():
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
-
-
-def test_segment_padded_storage_gives_the_oracle_order():
-    """hip.chunk_padded's layout folded the way the kernels fold it (chunk sums; a chunk's block is its first
-    column's; a new block starts a new partial) equals the oracle's block-and-chunk order bit for bit."""
-    import oracle
-    from helpers import random_csr
-    rng = np.random.default_rng(11)
-    M = random_csr(rng, 300, 413, 0.06, long_rows=(5, 200), long_len=400)
-    M = sp.vstack([M[:100], sp.csr_matrix((4, 413), dtype=np.float32), M[100:]]).tocsr()
-    M.sort_indices()
-    x = rng.standard_normal((2, 413)).astype(np.float32)
-    chunk_ptr, pidx, pw, counts = hip.chunk_padded(M)
-    width = hip.block_width(M.shape[1])
-    assert width == 52 and np.array_equal(np.diff(chunk_ptr), counts)
-    y = np.zeros((2, M.shape[0]), np.float32)
-    for b in range(2):
-        for r in range(M.shape[0]):
-            total = part = np.float32(0)
-            prev = -1
-            for c in range(chunk_ptr[r], chunk_ptr[r + 1]):
-                ids, ws = pidx[16 * c:16 * c + 16], pw[16 * c:16 * c + 16]
-                assert ids[0] >= 0 and len({int(i) // width for i in ids if i >= 0}) == 1
-                s = np.float32(0)
-                for i, wv in zip(ids, ws):
-                    if i >= 0:
-                        s = np.float32(s + np.float32(wv * x[b, i]))
-                blk = int(ids[0]) // width
-                if blk != prev:
-                    if prev >= 0:
-                        total = np.float32(total + part)
-                    part, prev = np.float32(0), blk
-                part = np.float32(part + s)
-            if prev >= 0:
-                total = np.float32(total + part)
-            y[b, r] = total
-    assert np.array_equal(y, oracle.spmv(M.indptr, M.indices, M.data, x))
-    assert (pidx >= 0).sum() == M.nnz

@@ -2,7 +2,7 @@
 
 The bar: integer, index and spike work bit-exact; the LIF, pull, optic and CSR kernels bit-exact too, because the
 oracle restates the same fp32 operation order (the reference CPU order for the LIF and the synaptic input, the
-defined block-and-chunk order for CSR products); CSR products additionally within 1e-5 relative of float64 scipy."""
+defined chunked order for CSR products); CSR products additionally within 1e-5 relative of float64 scipy."""
 import math
 
 import numpy as np
@@ -173,8 +173,10 @@ def test_pull_half_weights():
     assert_bits(N_(g), ref, "pull half")
 
 
-@pytest.mark.parametrize("B", [1, 3, 8])
-def test_csr_products_bit_exact_and_close_to_float64(B):
+@pytest.mark.parametrize("B,kernel,range_cols", [(1, "grouped", 0), (3, "grouped", 0), (8, "grouped", 0),
+                                                  (1, "chunked", 0), (3, "chunked", 0), (1, "lds", 0),
+                                                  (3, "lds", 1024), (2, "lds", 1500)])
+def test_csr_products_bit_exact_and_close_to_float64(B, kernel, range_cols):
     rng = np.random.default_rng(5 + B)
     rows, cols = 2000, 1700
     M = random_csr(rng, rows, cols, 0.03, long_rows=(0, 999, 1998), long_len=1650)
@@ -188,9 +190,10 @@ def test_csr_products_bit_exact_and_close_to_float64(B):
     x = rng.standard_normal((B, cols)).astype(np.float32)
     x[:, ::3] = 0
     ref = oracle.spmv(M.indptr, M.indices, M.data, x)
-    csr = hip.CSR(M, DEV)
+    csr = hip.CSR(M, DEV, kernel=kernel, range_cols=range_cols)
+    assert csr.kernel == kernel
     y = csr.matvec(T(x))
-    assert_bits(N_(y), ref, "grouped")
+    assert_bits(N_(y), ref, kernel)
     acc = T(rng.standard_normal((B, rows)).astype(np.float32))
     want = (N_(acc) + (np.float32(0.5) * ref).astype(np.float32)).astype(np.float32)
     assert_bits(N_(csr.matvec(T(x), out=acc, scale=0.5, accumulate=True)), want, "scale + accumulate")
