@@ -86,6 +86,56 @@ __device__ __forceinline__ float fold32_pair(ci32p idx, cf32p w, const float* __
     return total;
 }
 
+// fold32_pair over chunk-padded storage (row r's chunks [cptr[r], cptr[r+1]), chunk c = entries [16 c, 16 c + 16) of
+// pidx / pw, pads idx -1 and skipped): each half-wave loads its own row's chunk with 16-byte loads (the same
+// addresses on all its lanes) and gathers its 16 rows of x; the loads of chunk k + 1 are issued before chunk k is
+// folded (two buffers alternating).  Past a row's last chunk a half repeats that chunk's loads and folds nothing.
+// Same additions, same order as fold32_pair.
+struct HalfBuf {
+    int i[16];
+    float w[16];
+    float x[16];
+};
+
+__device__ __forceinline__ float fold32_pair_padded(const int4* __restrict__ pidx, const float4* __restrict__ pw,
+                                                    const float* __restrict__ x, int ca0, int ca1, int cb0, int cb1,
+                                                    int l, bool h) {
+    const int n0 = ca1 - ca0, n1 = cb1 - cb0, nmax = max(n0, n1);
+    const int c0 = h ? cb0 : ca0, nh = h ? n1 : n0;
+    auto load = [&](int k, HalfBuf& b) {
+        const int c = max(c0 + min(k, nh - 1), 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 iv = pidx[(int64_t)c * 4 + q];
+            const float4 wv = pw[(int64_t)c * 4 + q];
+            b.i[4 * q] = iv.x; b.i[4 * q + 1] = iv.y; b.i[4 * q + 2] = iv.z; b.i[4 * q + 3] = iv.w;
+            b.w[4 * q] = wv.x; b.w[4 * q + 1] = wv.y; b.w[4 * q + 2] = wv.z; b.w[4 * q + 3] = wv.w;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) b.x[j] = x[(size_t)max(b.i[j], 0) * NL + l];
+    };
+    float total = 0.0f;
+    auto fold = [&](int k, const HalfBuf& b) {
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (b.i[j] >= 0) { const float p = b.w[j] * b.x[j]; s = s + p; }
+        if (k < nh) total = total + s;
+    };
+    if (nmax == 0) return 0.0f;
+    HalfBuf A, B;
+    load(0, A);
+    for (int k = 0; k < nmax; k += 2) {
+        load(k + 1, B);
+        fold(k, A);
+        if (k + 1 < nmax) {
+            load(k + 2, A);
+            fold(k + 1, B);
+        }
+    }
+    return total;
+}
+
 // Per-unit coefficients and per-edge weights from the parameters.
 __global__ __launch_bounds__(256) void k_adj_params(const float* __restrict__ tau, const float* __restrict__ bias,
                                                     const int* __restrict__ unit_type, float* __restrict__ a,
@@ -114,7 +164,7 @@ __global__ __launch_bounds__(256) void k_adj_dr(const float* __restrict__ v, con
 }
 
 struct FwdArgs {
-    const int *ptr, *idx, *sched;
+    const int *ptr, *idx, *sched;       // chunk-padded: ptr in chunks, idx / w 16 entries per chunk
     const float *w, *a, *b;
     const float *v, *adapt, *dr, *P;     // step t
     float *v1, *adapt1, *dr1, *inp;      // step t+1 (inp of step t, may be null)
@@ -133,8 +183,8 @@ __global__ __launch_bounds__(256) void k_adj_forward(FwdArgs f) {
         const int r0 = f.sched[2 * pr];
         const int r1 = 2 * pr + 1 < f.rows ? f.sched[2 * pr + 1] : -1;
         const int r = h ? r1 : r0;
-        const float y = fold32_pair((ci32p)f.idx, (cf32p)f.w, f.dr, f.ptr[r0], f.ptr[r0 + 1], r1 < 0 ? 0 : f.ptr[r1],
-                                    r1 < 0 ? 0 : f.ptr[r1 + 1], l, h);
+        const float y = fold32_pair_padded((const int4*)f.idx, (const float4*)f.w, f.dr, f.ptr[r0], f.ptr[r0 + 1],
+                                           r1 < 0 ? 0 : f.ptr[r1], r1 < 0 ? 0 : f.ptr[r1 + 1], l, h);
         if (r < 0) continue;
         const int64_t q = (int64_t)r * NL + l;
         float inp = f.g_rr * y;
@@ -345,8 +395,8 @@ FH_EXPORT int launch_adj_dr(void** p, int n, void* stream) {
     FH_LAUNCH_CHECK();
 }
 
-// One forward step.  p = [ptr, idx, sched, w, a, b, v, adapt, dr, P, v1, adapt1, dr1, inp|null, readout,
-// target|null, loss|null]; f = [g_rr, g_ad, a_ad].
+// One forward step.  p = [chunk_ptr, pidx, sched, pw (chunk-padded W: 16 entries per chunk, pads idx -1), a, b, v,
+// adapt, dr, P, v1, adapt1, dr1, inp|null, readout, target|null, loss|null]; f = [g_rr, g_ad, a_ad].
 FH_EXPORT int launch_adj_forward(void** p, int rows, const float* f, int grid, void* stream) {
     if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
     if (!rows) return 0;

@@ -102,6 +102,11 @@ class OpticAdjoint:
         bins = (ut[rows] * K + ut[W0.indices]).astype(np.int32)
         self.bin = t(bins)
         self.sched = t(np.argsort(-np.diff(W0.indptr), kind="stable").astype(np.int32))
+        # the forward product's chunk-padded copy (hip.chunk_padded): weights of pads are 0 * gain
+        fptr, fidx, fw0, _ = hip.chunk_padded(W0)
+        fbin = np.zeros(len(fidx), np.int32)
+        fbin[fidx >= 0] = bins
+        self.fptr, self.fidx, self.fw0, self.fbin = t(fptr.astype(np.int32)), t(fidx), t(fw0), t(fbin)
         WT = W0.T.tocsr()
         WT.sort_indices()
         rowsT = np.repeat(np.arange(n), np.diff(WT.indptr))
@@ -123,6 +128,7 @@ class OpticAdjoint:
         z = lambda *s: torch.zeros(*s, device=dev)
         self.a, self.b = z(n), z(n)
         self.w, self.wT = z(self.nnz), z(self.nnz)
+        self.fw = z(len(fidx))
         self.a_ad = float(math.exp(-c.dt / c.adapt_tau))
         self.f_fwd = (ct.c_float * 3)(c.gain_rr, c.adapt_gain, self.a_ad)
         self.loss_parts = z(self.T, self.R)
@@ -154,10 +160,11 @@ class OpticAdjoint:
         self._call("launch_adj_params", [tau, bias, self.unit_type, self.a, self.b], self.n, float(self.c.dt))
         self._call("launch_adj_weights", [gain, self.bin, self.w0, self.w], self.nnz)
         self._call("launch_adj_weights", [gain, self.binT, self.w0T, self.wT], self.nnz)
+        self._call("launch_adj_weights", [gain, self.fbin, self.fw0, self.fw], self.fw.numel())
 
     def _step(self, t, v, ad, dr, v1, ad1, dr1, inp=None):
         self._call("launch_adj_forward",
-                   [self.ptr, self.idx, self.sched, self.w, self.a, self.b, v, ad, dr, self.P[t], v1, ad1, dr1, inp,
+                   [self.fptr, self.fidx, self.sched, self.fw, self.a, self.b, v, ad, dr, self.P[t], v1, ad1, dr1, inp,
                     self.readout, self.target[t], self.loss_parts[t]],
                    self.n, self.f_fwd, self.grid)
 
