@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_adj_back_ly(const float* __restrict__ l
 }
 
 struct BackArgs {
-    const int *ptrT, *idxT, *schedT;
+    const int *ptrT, *idxT, *schedT;     // chunk-padded W^T: ptrT in chunks, idxT / wT 16 entries per chunk
     const float *wT, *a, *b;
     const float *ly, *v, *dr;            // step t
     float *lv, *la;                      // in: step t+1, out: step t (own rows only)
@@ -261,8 +261,8 @@ __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
         const int j0 = k.schedT[2 * pr];
         const int j1 = 2 * pr + 1 < k.rows ? k.schedT[2 * pr + 1] : -1;
         const int j = h ? j1 : j0;
-        const float s = fold32_pair((ci32p)k.idxT, (cf32p)k.wT, k.ly, k.ptrT[j0], k.ptrT[j0 + 1],
-                                    j1 < 0 ? 0 : k.ptrT[j1], j1 < 0 ? 0 : k.ptrT[j1 + 1], l, h);
+        const float s = fold32_pair_padded((const int4*)k.idxT, (const float4*)k.wT, k.ly, k.ptrT[j0],
+                                           k.ptrT[j0 + 1], j1 < 0 ? 0 : k.ptrT[j1], j1 < 0 ? 0 : k.ptrT[j1 + 1], l, h);
         if (j < 0) continue;
         const int64_t q = (int64_t)j * NL + l;
         const float la1 = k.la[q], lv1 = k.lv[q];
@@ -422,7 +422,8 @@ FH_EXPORT int launch_adj_back_ly(void** p, int rows, float g_rr, void* stream) {
     FH_LAUNCH_CHECK();
 }
 
-// p = [ptrT, idxT, schedT, wT, a, b, ly, v, dr, lv, la, readout, target|null, gb]; f = [g_ad, a_ad, loss_scale].
+// p = [ptrT, idxT, schedT, wT (chunk-padded W^T as launch_adj_forward's W), a, b, ly, v, dr, lv, la, readout,
+// target|null, gb]; f = [g_ad, a_ad, loss_scale].
 FH_EXPORT int launch_adj_back_rows(void** p, int rows, const float* f, int grid, void* stream) {
     if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
     if (!rows) return 0;

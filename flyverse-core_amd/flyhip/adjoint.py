@@ -100,7 +100,6 @@ class OpticAdjoint:
         self.ptr, self.idx = t(W0.indptr.astype(np.int32)), t(W0.indices.astype(np.int32))
         self.w0 = t(W0.data)
         bins = (ut[rows] * K + ut[W0.indices]).astype(np.int32)
-        self.bin = t(bins)
         self.sched = t(np.argsort(-np.diff(W0.indptr), kind="stable").astype(np.int32))
         # the forward product's chunk-padded copy (hip.chunk_padded): weights of pads are 0 * gain
         fptr, fidx, fw0, _ = hip.chunk_padded(W0)
@@ -110,9 +109,12 @@ class OpticAdjoint:
         WT = W0.T.tocsr()
         WT.sort_indices()
         rowsT = np.repeat(np.arange(n), np.diff(WT.indptr))
-        self.ptrT, self.idxT = t(WT.indptr.astype(np.int32)), t(WT.indices.astype(np.int32))
-        self.w0T = t(WT.data.astype(np.float32))
-        self.binT = t((ut[WT.indices] * K + ut[rowsT]).astype(np.int32))
+        binsT = (ut[WT.indices] * K + ut[rowsT]).astype(np.int32)
+        # the transposed product's chunk-padded copy
+        bptr, bidx, bw0, _ = hip.chunk_padded(WT)
+        bbin = np.zeros(len(bidx), np.int32)
+        bbin[bidx >= 0] = binsT
+        self.bptr, self.bidx, self.bw0, self.bbin = t(bptr.astype(np.int32)), t(bidx), t(bw0), t(bbin)
         self.schedT = t(np.argsort(-np.diff(WT.indptr), kind="stable").astype(np.int32))
         self.nnz = W0.nnz
         pg, sg = _segments(bins.astype(np.int64), K * K)
@@ -127,8 +129,8 @@ class OpticAdjoint:
         self.target = torch.as_tensor(target, dtype=torch.float32, device=dev).contiguous()
         z = lambda *s: torch.zeros(*s, device=dev)
         self.a, self.b = z(n), z(n)
-        self.w, self.wT = z(self.nnz), z(self.nnz)
         self.fw = z(len(fidx))
+        self.bw = z(len(bidx))
         self.a_ad = float(math.exp(-c.dt / c.adapt_tau))
         self.f_fwd = (ct.c_float * 3)(c.gain_rr, c.adapt_gain, self.a_ad)
         self.loss_parts = z(self.T, self.R)
@@ -158,9 +160,8 @@ class OpticAdjoint:
             raise ValueError("tau and bias need one value per type, gain one per type pair")
         self._params = (tau, bias, gain)
         self._call("launch_adj_params", [tau, bias, self.unit_type, self.a, self.b], self.n, float(self.c.dt))
-        self._call("launch_adj_weights", [gain, self.bin, self.w0, self.w], self.nnz)
-        self._call("launch_adj_weights", [gain, self.binT, self.w0T, self.wT], self.nnz)
         self._call("launch_adj_weights", [gain, self.fbin, self.fw0, self.fw], self.fw.numel())
+        self._call("launch_adj_weights", [gain, self.bbin, self.bw0, self.bw], self.bw.numel())
 
     def _step(self, t, v, ad, dr, v1, ad1, dr1, inp=None):
         self._call("launch_adj_forward",
@@ -252,7 +253,7 @@ class OpticAdjoint:
     def _back_rows(self, k, t, f_back):
         target = self.target[t - 1] if t >= 1 else None
         self._call("launch_adj_back_rows",
-                   [self.ptrT, self.idxT, self.schedT, self.wT, self.a, self.b, self.ly, self.S_v[k], self.S_dr[k],
+                   [self.bptr, self.bidx, self.schedT, self.bw, self.a, self.b, self.ly, self.S_v[k], self.S_dr[k],
                     self.lv, self.la, self.readout, target, self.gb],
                    self.n, f_back, self.grid)
 
