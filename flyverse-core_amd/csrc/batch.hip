@@ -213,6 +213,52 @@ __device__ __forceinline__ float add_bits(float acc, uint32_t bits, int wi, int 
     return acc;
 }
 
+// Mode 5's form of the transpose: the same five exchange stages, each merging the kept and received bits with one
+// bit-field select (v_bitop3 / v_bfi from the xor form) instead of an and plus an and-or.
+template <int S, uint32_t M>
+__device__ __forceinline__ uint32_t transpose_stage_x(uint32_t x, int lane) {
+    const uint32_t t = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (S << 10) | 0x1F);
+    const bool lower = (lane & S) != 0;
+    const uint32_t ts = __builtin_amdgcn_alignbit(t, t, lower ? S : 32 - S);
+    const uint32_t keep = lower ? ~M : M;
+    return ts ^ ((x ^ ts) & keep);
+}
+
+__device__ __forceinline__ void transpose_chunk_x(uint64_t mj, int lane, uint32_t& lo, uint32_t& hi) {
+    const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)mj, (uint32_t)(mj >> 32), false, false);
+    uint32_t x = sw[0], y = sw[1];
+    x = transpose_stage_x<16, 0x0000FFFFu>(x, lane); y = transpose_stage_x<16, 0x0000FFFFu>(y, lane);
+    x = transpose_stage_x<8, 0x00FF00FFu>(x, lane);  y = transpose_stage_x<8, 0x00FF00FFu>(y, lane);
+    x = transpose_stage_x<4, 0x0F0F0F0Fu>(x, lane);  y = transpose_stage_x<4, 0x0F0F0F0Fu>(y, lane);
+    x = transpose_stage_x<2, 0x33333333u>(x, lane);  y = transpose_stage_x<2, 0x33333333u>(y, lane);
+    x = transpose_stage_x<1, 0x55555555u>(x, lane);  y = transpose_stage_x<1, 0x55555555u>(y, lane);
+    lo = x;
+    hi = y;
+}
+
+// Index of the lowest set bit, -1 for a zero word (v_ffbl_b32 as the hardware defines it).
+__device__ __forceinline__ int lowest_bit(uint32_t x) {
+    int r;
+    __asm__("v_ffbl_b32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Mode 5's adds of one 32-edge half of a chunk: seg points at the half's 32 weights in LDS and seg[-1] holds -0.0,
+// so a lane with no bit left reads -0.0 and its add leaves acc unchanged bit for bit (x + -0.0 = x for every x,
+// signed zeros included).  The adds therefore need no select; a lane's adds still go over its set bits in
+// ascending edge order, two per round.
+__device__ __forceinline__ float add_bits_lds(float acc, uint32_t bits, const float* seg) {
+    while (__builtin_amdgcn_ballot_w64(bits != 0u)) {
+        const uint32_t b1 = bits & (bits - 1u);
+        const float w0 = seg[lowest_bit(bits)];
+        const float w1 = seg[lowest_bit(b1)];
+        acc = acc + w0;
+        acc = acc + w1;
+        bits = b1 & (b1 - 1u);
+    }
+    return acc;
+}
+
 struct BatchLif {
     const int* ptr;          // (n_rows + 1) post-major CSR over LIF rows
     const uint32_t* pre_off; // (nnz) byte offset of the presynaptic row's masks in the ring (row * ring_slots * 8)
@@ -235,10 +281,17 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, int K) {
-    __shared__ float wstage[4][64];
+    constexpr bool L5 = MODE == 5;                  // LDS-staged weights
+    __shared__ float wstage[MODE == 2 ? 4 : 1][64];
     __shared__ uint64_t cmask[MODE == 4 ? 4 : 1][256];
     __shared__ float cweight[MODE == 4 ? 4 : 1][256];
+    // mode 5: per wavefront 16 segments of 33 floats, [-0.0, the weights of 32 consecutive edges]; segments 2c and
+    // 2c + 1 hold the two halves of chunk c of the row (rows of up to 512 edges) or of the current work item
+    __shared__ float wseg[L5 ? 4 : 1][L5 ? 16 * 33 : 1];
     const int lane = threadIdx.x & 63;
+    float* const ws = wseg[L5 ? threadIdx.x >> 6 : 0];
+    const int ws_lane = (lane >> 5) * 33 + 1 + (lane & 31);      // this lane's weight slot within a chunk's segments
+    if (L5 && lane < 16) ws[lane * 33] = -0.0f;
     const int gw = blockIdx.x * 4 + wave_id();
     const int nw = gridDim.x * 4;
     const uint32_t T0 = *a.engine_step + (uint32_t)step_offset;
@@ -349,6 +402,28 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                             acc = fold_chunk_permute(acc, mj, cw[cb + lane], lane);
                         }
                         __builtin_amdgcn_wave_barrier();
+                    } else if (L5) {
+                        uint32_t lo[4], hi[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (u < nch) transpose_chunk_x(base + u * 64 < ne ? m[u] : 0ull, lane, lo[u], hi[u]);
+                        const float* sg = ws + 1;
+                        if (G > 2) {
+                            // weights of this item's chunks, in the item's segments (the previous item's reads of
+                            // them were issued earlier by this wavefront, and LDS serves a wavefront in order)
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) ws[u * 66 + ws_lane] = wv[u];
+                            __builtin_amdgcn_wave_barrier();
+                        } else {
+                            sg += c.grp * 8 * 33;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (u < nch) {
+                                acc = add_bits_lds(acc, lo[u], sg + (2 * u) * 33);
+                                acc = add_bits_lds(acc, hi[u], sg + (2 * u + 1) * 33);
+                            }
+                        }
                     } else {
                         uint32_t lo[4], hi[4];
 #pragma unroll
@@ -398,6 +473,14 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                         const int e = min(e0 + c * 64 + lane, e1 - 1);
                         oc[c] = a.pre_off[e];
                         wc[c] = a.w[e];
+                    }
+                    if (L5) {
+                        // the row's weights go to LDS once per launch (after this wavefront's reads of the previous
+                        // row's, in program order)
+                        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) ws[c * 66 + ws_lane] = wc[c];
+                        __builtin_amdgcn_wave_barrier();
                     }
                     run([&](const Cursor& c, uint32_t* o, float* wv) {
                         const bool hi = c.grp != 0;
@@ -676,7 +759,8 @@ inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
 // refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12)];
 // dims = [n_rows, lanes, ring_slots, delay, record, grid, mode, row_major]; mode 0 reads the spike masks through the
 // scalar cache, mode 1 gathers them with vector loads, mode 2 gathers and transposes them, mode 3 transposes with
-// the gathers of the next work item in flight, mode 4 also drops the edges no fly transmits before transposing
+// the gathers of the next work item in flight, mode 4 also drops the edges no fly transmits before transposing,
+// mode 5 is mode 3 with the weights staged in LDS behind a -0.0 sentinel (adds without selects)
 // (fold_edges, fold_edges_gather, fold_edges_transposed, fold_chunk_permute: the same additions in the same order,
 // so the same result).
 FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream) {
@@ -692,7 +776,7 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     a.mode = dims[6];
     a.row_major = dims[7];
     if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
-        grid <= 0 || a.mode < 0 || a.mode > 4)
+        grid <= 0 || a.mode < 0 || a.mode > 5)
         return FH_ERR_ARGS;
     if (a.n_rows == 0 || K == 0) return 0;
     switch (a.mode) {
@@ -700,7 +784,8 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
         case 1: hipLaunchKernelGGL(k_batch_lif<1>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 2: hipLaunchKernelGGL(k_batch_lif<2>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 3: hipLaunchKernelGGL(k_batch_lif<3>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
-        default: hipLaunchKernelGGL(k_batch_lif<4>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
+        case 4: hipLaunchKernelGGL(k_batch_lif<4>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
+        default: hipLaunchKernelGGL(k_batch_lif<5>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
     }
     FH_LAUNCH_CHECK();
 }

@@ -29,7 +29,8 @@ def rollouts(layout, n, frames_lo, frames_hi, frame_steps, seed=0):
     return out
 
 
-@pytest.mark.parametrize("graph,mode", [(False, 0), (True, 1), (True, 2), (False, 2), (True, 3), (False, 3), (True, 4), (False, 4)])
+@pytest.mark.parametrize("graph,mode", [(False, 0), (True, 1), (True, 2), (False, 2), (True, 3), (False, 3), (True, 4), (False, 4),
+                                        (True, 5), (False, 5)])
 def test_lanes_equal_solo_oracle_runs_with_refill(graph, mode):
     conn, retina, layout, lp, op = fly_setup(0.02, seed=2)
     frame_ms, frame_steps = 1.0, 10           # 10 LIF steps and one optic substep per frame: the ring holds a frame
@@ -65,7 +66,7 @@ def test_lanes_equal_solo_oracle_runs_with_refill(graph, mode):
 
 
 def test_full_scale_batch_is_deterministic_and_matches_plain_path():
-    """Full-scale graph, 64 flies, 3 frames of 10 ms: two runs (fold modes 2 and 4) give the same bits, and lane 7
+    """Full-scale graph, 64 flies, 3 frames of 10 ms: three runs (fold modes 2, 4 and 5) give the same bits, and lane 7
     equals the plain single-fly FlyBrain (batch=1, same seeds and inputs) bit for bit."""
     from flyhip.brain import LIFParams
     from flyhip.fly import FlyBrain
@@ -75,7 +76,7 @@ def test_full_scale_batch_is_deterministic_and_matches_plain_path():
     op = OpticParams(tau_by_type=conn.rate_tau, baseline_by_type=conn.rate_baseline, pair_gain=[])
     rs = rollouts(layout, 64, 3, 3, 100, seed=5)
     states = []
-    for rep, mode in enumerate((2, 4)):
+    for rep, mode in enumerate((2, 4, 5)):
         bb = BatchBrain(conn, retina, lp, op, device=DEV, fold_mode=mode)
         bb.submit(rs)
         bb.run(3)
@@ -86,8 +87,9 @@ def test_full_scale_batch_is_deterministic_and_matches_plain_path():
             optic7 = bb.optic_state(7)
             total = float(bb.counts.sum())
         del bb
-    for k in states[0]:
-        assert torch.equal(states[0][k], states[1][k]), k
+    for other in states[1:]:
+        for k in states[0]:
+            assert torch.equal(states[0][k], other[k]), k
     assert total > 0
     r = rs[7]
     fb = FlyBrain(conn, retina, batch=1, device=DEV, seed=r.seed, lif_params=lp, optic_params=op)
