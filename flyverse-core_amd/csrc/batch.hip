@@ -522,74 +522,102 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
 
 // ------------------------------------------------------------------ optic lobe, fly-minor
 // Chunked products over a (cols, 64) input with chunk-padded storage: row r owns chunks [chunk_ptr[r],
-// chunk_ptr[r+1]), chunk c is entries [16 c, 16 c + 16) of pidx / pw, pads have idx -1 and are skipped.  A wavefront
-// walks the chunks of its rows (sched order, stride = number of wavefronts) as one stream: the scalar loads of a
-// chunk's indices and weights and its 16 row gathers (256 bytes each, one per entry) are issued one chunk ahead, so
-// the fold of chunk c overlaps the loads of chunk c + 1.  Each chunk is folded from +0 in entry order and the chunk
-// sums of a row in chunk order (csrc/spmv.hip order); finish(row, total) consumes the row total.
-struct FmBuf {
-    uint32_t idx[16];
-    float w[16];
-    float x[16];
-};
-
+// chunk_ptr[r+1]) (at least one: an empty row has one all-pad chunk), chunk c is entries [16 c, 16 c + 16) of
+// pidx / pw, pads have idx -1 and are skipped.  A wavefront walks the chunks of its rows (sched order, stride =
+// number of wavefronts) as one stream, three chunks deep: the indices and weights of chunk c + 2 arrive by one vector
+// load (lanes 0-15 the indices, lanes 16-31 the weights), the 16 row gathers of chunk c + 1 (256 bytes each) are in
+// flight, and chunk c is folded.  Vector loads complete in issue order, so each wait leaves the younger stages' loads
+// outstanding.  The row after the leading cursor's is fetched (scalar loads) when the cursor enters a row, so moving
+// to the next row waits on nothing.  Each chunk is folded from +0 in entry order and the chunk sums of a row in chunk
+// order (csrc/spmv.hip order); finish(row, total) consumes the row total.
 struct FmCursor { int si, r, c, ce; };
 
 template <class Finish>
-__device__ __forceinline__ void fm_chunk_stream(const int* __restrict__ chunk_ptr, cu32p pidx, cf32p pw,
-                                                const int* __restrict__ sched, const float* __restrict__ x, int rows,
-                                                int lane, Finish finish) {
+__device__ __forceinline__ void fm_chunk_stream(const int* __restrict__ chunk_ptr, const int* __restrict__ pidx,
+                                                const float* __restrict__ pw, const int* __restrict__ sched,
+                                                const float* __restrict__ x, int rows, int lane, Finish finish) {
     const int nw = gridDim.x * 4;
-    // move to the first non-empty row at or after schedule index si (empty rows are finished with a zero total)
-    auto seek = [&](FmCursor& q, int si) {
+    auto row_at = [&](int si) {
+        FmCursor q;
         q.si = si;
-        while (q.si < rows) {
-            q.r = sched[q.si];
-            q.c = chunk_ptr[q.r];
-            q.ce = chunk_ptr[q.r + 1];
-            if (q.c < q.ce) return;
-            finish(q.r, 0.0f);
-            q.si += nw;
-        }
-    };
-    auto next = [&](FmCursor q) {
-        if (q.c + 1 < q.ce) { ++q.c; return q; }
-        seek(q, q.si + nw);
+        q.r = sched[min(si, rows - 1)];
+        q.c = chunk_ptr[q.r];
+        q.ce = chunk_ptr[q.r + 1];
         return q;
     };
-    auto load = [&](const FmCursor& q, FmBuf& b) {
-        const int c = q.c;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) { b.idx[j] = pidx[c * 16 + j]; b.w[j] = pw[c * 16 + j]; }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) b.x[j] = x[(size_t)max((int)b.idx[j], 0) * 64 + lane];
+    FmCursor q0 = row_at(blockIdx.x * 4 + wave_id());
+    if (q0.si >= rows) return;
+    FmCursor nr = row_at(q0.si + nw);                  // the row after the leading cursor's
+    // the leading cursor's next chunk (past the last row it stays past the end: si >= rows)
+    auto next = [&](FmCursor q) {
+        if (q.c + 1 < q.ce) { ++q.c; return q; }
+        if (q.si >= rows) return q;
+        q = nr;
+        nr = row_at(min(q.si + nw, rows));
+        return q;
     };
+    const int vl = lane & 31;
+    // one chunk's indices (lanes 0-15) and weights (lanes 16-31; lanes 32-63 repeat lanes 0-31)
+    auto load_v = [&](const FmCursor& q) -> uint32_t {
+        const uint32_t* src = vl < 16 ? (const uint32_t*)pidx + q.c * 16 + vl
+                                      : (const uint32_t*)pw + q.c * 16 + (vl - 16);
+        return *src;
+    };
+    // the chunk's 16 row gathers (pads read row 0), its weights read out into scalars and the mask of its real
+    // entries, so the index/weight register is free once the gathers are issued
+    struct Stage {
+        float x[16];
+        float w[16];
+        uint32_t valid;
+    };
+    auto load_x = [&](uint32_t V, Stage& st) {
+        st.valid = (uint32_t)__builtin_amdgcn_ballot_w64((int)V >= 0) & 0xFFFFu;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int ij = __builtin_amdgcn_readlane((int)V, j);
+            st.w[j] = __int_as_float(__builtin_amdgcn_readlane((int)V, 16 + j));
+            st.x[j] = x[(size_t)max(ij, 0) * 64 + lane];
+        }
+    };
+    // the barriers keep the issue order load_v, load_x, fold (the waits depend on it): pin() stops the compiler
+    // from sinking a load below it, the scheduling barrier stops the instruction scheduler from moving code across
+    auto pin = [] { __asm__ volatile("" ::: "memory"); __builtin_amdgcn_sched_barrier(0); };
     float total = 0.0f;
-    auto fold = [&](const FmCursor& q, const FmBuf& b) {
+    auto fold = [&](const FmCursor& q, const Stage& st) {
         float s = 0.0f;
 #pragma unroll
         for (int j = 0; j < 16; ++j)
-            if ((int)b.idx[j] >= 0) { const float p = b.w[j] * b.x[j]; s = s + p; }
+            if ((st.valid >> j) & 1u) { const float p = st.w[j] * st.x[j]; s = s + p; }
         total = total + s;
         if (q.c + 1 == q.ce) { finish(q.r, total); total = 0.0f; }
     };
-    FmCursor p0;
-    seek(p0, blockIdx.x * 4 + wave_id());
-    if (p0.si >= rows) return;
-    FmBuf A, B;
-    load(p0, A);
+    // stages past the end repeat chunk q0's loads and are never folded; two of each buffer alternate, so no register
+    // that a load is still writing is ever copied
+    // (the prologue issues the loads in the order the loop does, V of q1 before the gathers of q0, so the loop is
+    // entered with the same loads outstanding as around its back edge)
+    FmCursor q1 = next(q0);
+    Stage SA, SB;
+    const uint32_t V0 = load_v(q0);
+    uint32_t VA, VB = load_v(q1.si < rows ? q1 : q0);
+    pin();
+    load_x(V0, SA);
+    pin();
+    FmCursor q2 = next(q1);
     for (;;) {
-        const FmCursor p1 = next(p0);
-        const bool more1 = p1.si < rows;
-        load(more1 ? p1 : p0, B);                       // past the end: a repeat that is never folded
-        fold(p0, A);
-        if (!more1) break;
-        const FmCursor p2 = next(p1);
-        const bool more2 = p2.si < rows;
-        load(more2 ? p2 : p1, A);
-        fold(p1, B);
-        if (!more2) break;
-        p0 = p2;
+        VA = load_v(q2.si < rows ? q2 : q0);
+        pin();
+        load_x(VB, SB);
+        pin();
+        fold(q0, SA);
+        if (q1.si >= rows) break;
+        q0 = q1; q1 = q2; q2 = next(q2);
+        VB = load_v(q2.si < rows ? q2 : q0);
+        pin();
+        load_x(VA, SA);
+        pin();
+        fold(q0, SB);
+        if (q1.si >= rows) break;
+        q0 = q1; q1 = q2; q2 = next(q2);
     }
 }
 
@@ -599,14 +627,38 @@ __global__ __launch_bounds__(256) void k_spmm_fm(const int* __restrict__ chunk_p
                                                  const float* __restrict__ x, float* __restrict__ y, int rows,
                                                  float scale) {
     const int lane = threadIdx.x & 63;
-    fm_chunk_stream(chunk_ptr, (cu32p)pidx, (cf32p)pw, sched, x, rows, lane, [&](int r, float total) {
+    fm_chunk_stream(chunk_ptr, pidx, pw, sched, x, rows, lane, [&](int r, float total) {
         if (scale != 1.0f) total = total * scale;
         y[(int64_t)r * 64 + lane] = total;
     });
 }
 
-// One optic substep fused with its recurrent product: y = W_rr dr_cur (chunked order), then the update of
-// flyverse/optic.py:525-551 (neural.cu:68-79) writing the next dr into dr_next; lanes of finished flies only copy.
+// The optic update of flyverse/optic.py:525-551 (neural.cu:68-79) for row r, lane q = r * 64 + lane, given the
+// recurrent product total = (W_rr dr_cur)[r]: writes v, adapt and the next dr; lanes of finished flies only copy dr.
+__device__ __forceinline__ void optic_update_one(int r, int64_t q, float total, bool live, const float* dr_cur,
+                                                 float* dr_next, float* v, float* adapt, const float* pr,
+                                                 const float* spk, const float* a, const float* b, float gr, float ga,
+                                                 float aad) {
+    const float old = dr_cur[q];
+    if (!live) { dr_next[q] = old; return; }
+    float inp = gr * total;
+    inp = inp + pr[q];
+    float gad = ga * adapt[q];
+    inp = inp - gad;
+    inp = inp + spk[q];
+    float d = v[q] - inp;
+    d = d * a[r];
+    float vn = inp + d;
+    v[q] = vn;
+    float da = adapt[q] - old;
+    da = da * aad;
+    adapt[q] = old + da;
+    float s2 = vn + b[r];
+    dr_next[q] = fh_clip01(s2) - b[r];
+}
+
+// One optic substep fused with its recurrent product: y = W_rr dr_cur (chunked order), then the update, writing the
+// next dr into dr_next.
 __global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict__ chunk_ptr, const int* __restrict__ pidx,
                                                           const float* __restrict__ pw, const int* __restrict__ sched,
                                                           const float* __restrict__ dr_cur, float* __restrict__ dr_next,
@@ -617,24 +669,9 @@ __global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict_
                                                           float ga, float aad) {
     const int lane = threadIdx.x & 63;
     const bool live = (*live_mask >> lane) & 1ull;
-    fm_chunk_stream(chunk_ptr, (cu32p)pidx, (cf32p)pw, sched, dr_cur, rows, lane, [&](int r, float total) {
-        const int64_t q = (int64_t)r * 64 + lane;
-        const float old = dr_cur[q];
-        if (!live) { dr_next[q] = old; return; }
-        float inp = gr * total;
-        inp = inp + pr[q];
-        float gad = ga * adapt[q];
-        inp = inp - gad;
-        inp = inp + spk[q];
-        float d = v[q] - inp;
-        d = d * a[r];
-        float vn = inp + d;
-        v[q] = vn;
-        float da = adapt[q] - old;
-        da = da * aad;
-        adapt[q] = old + da;
-        float s2 = vn + b[r];
-        dr_next[q] = fh_clip01(s2) - b[r];
+    fm_chunk_stream(chunk_ptr, pidx, pw, sched, dr_cur, rows, lane, [&](int r, float total) {
+        optic_update_one(r, (int64_t)r * 64 + lane, total, live, dr_cur, dr_next, v, adapt, pr, spk, a, b, gr, ga,
+                         aad);
     });
 }
 
