@@ -11,8 +11,9 @@ optic.py:82-87, 251-267, 343-347):
     loss = mean over t = 1..T, movies and readout units of (clip(v_t + b, 0, 1) - target_t)^2
 
 `OpticAdjoint.loss(tau, bias, gain)` runs the forward (FP32, csrc/adjoint.hip) keeping (v, adapt) every
-`checkpoint` steps; `gradients()` walks the segments backwards, recomputing each segment's states from its
-checkpoint and running the adjoint step by step: the transpose product (W^T applied to g_rr lambda_inp through the
+`checkpoint` steps and every state of the last segment; `gradients()` walks the segments backwards, recomputing each
+earlier segment's states from its checkpoint (with checkpoint >= T nothing is recomputed: the trajectory, about
+46 MB per step at full scale, stays in HBM) and running the adjoint step by step: the transpose product (W^T applied to g_rr lambda_inp through the
 transposed CSR), the clamp gates (gradient passes where 0 <= v + b <= 1, torch's convention for clamp), and a
 per-edge accumulation of sum_b lambda_y dr for the pair gains.  Per-type and per-pair sums are fixed-order segmented
 reductions, so the gradient is deterministic.  `OpticLossFn` wraps both in a torch.autograd.Function.
@@ -73,9 +74,11 @@ class OpticConstants:
 
 class OpticAdjoint:
     """The C4-style training problem on the GPU.  W0: (n, n) normalised signed weights (post, pre); unit_type (n,)
-    in [0, K); P: (T, n, 32) held input drive per step; target: (T, R, 32) for steps 1..T; readout: (R,) units."""
+    in [0, K); P: (T, n, 32) held input drive per step; target: (T, R, 32) for steps 1..T; readout: (R,) units.
+    checkpoint: steps per recomputed segment; None keeps the whole trajectory (4 * T * n * 128 bytes) and recomputes
+    nothing."""
 
-    def __init__(self, W0: sp.spmatrix, unit_type, n_types: int, P, target, readout, consts=None, checkpoint=20,
+    def __init__(self, W0: sp.spmatrix, unit_type, n_types: int, P, target, readout, consts=None, checkpoint=None,
                  device="cuda", grid=None):
         self.device = dev = torch.device(device)
         if not hip.is_rocm_gpu(dev):
@@ -94,7 +97,9 @@ class OpticAdjoint:
         self.R = len(readout)
         if tuple(target.shape) != (self.T, self.R, NL):
             raise ValueError(f"target must be (T, R, {NL})")
-        self.checkpoint = int(checkpoint)
+        self.checkpoint = self.T if checkpoint is None else int(checkpoint)
+        if self.checkpoint < 1:
+            raise ValueError("checkpoint must be >= 1")
         t = lambda a, dt=None: torch.as_tensor(np.ascontiguousarray(a), device=dev, dtype=dt)
         rows = np.repeat(np.arange(n), np.diff(W0.indptr))
         self.ptr, self.idx = t(W0.indptr.astype(np.int32)), t(W0.indices.astype(np.int32))
@@ -144,6 +149,7 @@ class OpticAdjoint:
         self.lv, self.la, self.ly = z(n, NL), z(n, NL), z(n, NL)
         self.ga, self.gb, self.eg = z(n), z(n), z(self.nnz)
         self._params = None
+        self._last_segment_ready = False
 
     # ------------------------------------------------------------------ launches
     def _call(self, name, tensors, *args):
@@ -173,17 +179,30 @@ class OpticAdjoint:
     def loss(self, tau, bias, gain) -> torch.Tensor:
         """Forward over T steps; returns the loss (a 1-element device tensor) and keeps the checkpoints."""
         self._set_params(tau, bias, gain)
-        n = self.n
+        n, C = self.n, self.checkpoint
         v = [torch.zeros(n, NL, device=self.device) for _ in range(2)]
         ad = [torch.zeros(n, NL, device=self.device) for _ in range(2)]
         dr = [torch.empty(n, NL, device=self.device) for _ in range(2)]
         self._call("launch_adj_dr", [v[0], self.b, dr[0]], n)
+        # the last segment's steps write straight into the segment buffers (with their inputs), so gradients() starts
+        # without recomputing it; with checkpoint >= T that is the whole trajectory and nothing is recomputed
+        t_last = ((self.T - 1) // C) * C
         for t in range(self.T):
             cur, nxt = t % 2, (t + 1) % 2
-            if t % self.checkpoint == 0:
-                self.ck_v[t // self.checkpoint].copy_(v[cur])
-                self.ck_a[t // self.checkpoint].copy_(ad[cur])
-            self._step(t, v[cur], ad[cur], dr[cur], v[nxt], ad[nxt], dr[nxt])
+            if t % C == 0:
+                self.ck_v[t // C].copy_(v[cur])
+                self.ck_a[t // C].copy_(ad[cur])
+            if t < t_last:
+                self._step(t, v[cur], ad[cur], dr[cur], v[nxt], ad[nxt], dr[nxt])
+                continue
+            k = t - t_last
+            if k == 0:
+                self.S_v[0].copy_(v[cur])
+                self.S_a[0].copy_(ad[cur])
+                self._call("launch_adj_dr", [self.S_v[0], self.b, self.S_dr[0]], n)
+            self._step(t, self.S_v[k], self.S_a[k], self.S_dr[k], self.S_v[k + 1], self.S_a[k + 1], self.S_dr[k + 1],
+                       self.S_inp[k])
+        self._last_segment_ready = True
         self._call("launch_loss_total", [self.loss_parts, self.loss_out], self.T * self.R,
                    1.0 / (self.T * NL * self.R))
         return self.loss_out
@@ -221,13 +240,15 @@ class OpticAdjoint:
         first = True
         for s in reversed(range(nseg)):
             t0, t1 = s * C, min(T, (s + 1) * C)
-            # recompute the segment's states from its checkpoint (the same launches as the forward: same bits)
-            self.S_v[0].copy_(self.ck_v[s])
-            self.S_a[0].copy_(self.ck_a[s])
-            self._call("launch_adj_dr", [self.S_v[0], self.b, self.S_dr[0]], n)
-            for k in range(t1 - t0):
-                self._step(t0 + k, self.S_v[k], self.S_a[k], self.S_dr[k], self.S_v[k + 1], self.S_a[k + 1],
-                           self.S_dr[k + 1], self.S_inp[k])
+            if not (first and self._last_segment_ready):
+                # recompute the segment's states from its checkpoint (the same launches as the forward: same bits)
+                self.S_v[0].copy_(self.ck_v[s])
+                self.S_a[0].copy_(self.ck_a[s])
+                self._call("launch_adj_dr", [self.S_v[0], self.b, self.S_dr[0]], n)
+                for k in range(t1 - t0):
+                    self._step(t0 + k, self.S_v[k], self.S_a[k], self.S_dr[k], self.S_v[k + 1], self.S_a[k + 1],
+                               self.S_dr[k + 1], self.S_inp[k])
+            self._last_segment_ready = False
             if first:
                 # the loss term of the last step, T: lambda_v_T = loss_scale (r_T - target_T) gate_T
                 self.ly.zero_()

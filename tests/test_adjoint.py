@@ -1,8 +1,8 @@
 """The optic-lobe adjoint.  CPU: the float64 torch restatement (oracle/optic_grad.py) is pinned by central finite
 differences.  GPU: the FP32 forward + hand-written adjoint (flyhip.adjoint) against float64 autograd of that
 restatement on the same inputs: loss within 1e-5 relative, every gradient group within 1e-4 relative (norm of the
-difference over the norm of the float64 gradient); checkpoint spacing does not change a bit; the autograd wrapper
-returns the same gradients."""
+difference over the norm of the float64 gradient); checkpoint spacing (or keeping the whole trajectory) does not
+change a bit, nor does a repeated backward; the autograd wrapper returns the same gradients."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -59,7 +59,7 @@ def rel(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("checkpoint", [1, 7, 24])
+@pytest.mark.parametrize("checkpoint", [1, 7, 24, None])
 def test_adjoint_matches_float64_autograd(checkpoint):
     from flyhip.adjoint import OpticAdjoint
     W0, ut, P, target, readout, tau, bias, gain = small_problem()
@@ -79,9 +79,11 @@ def test_adjoint_is_deterministic_and_checkpoint_invariant():
     W0, ut, P, target, readout, tau, bias, gain = small_problem(seed=3)
     t = lambda x: torch.as_tensor(x, device="cuda")
     results = []
-    for ck in (5, 5, 24):
+    for ck in (5, 5, 24, None, 7):
         prob = OpticAdjoint(W0, ut, 5, P, target, readout, checkpoint=ck, device="cuda")
         L = prob.loss(t(tau), t(bias), t(gain)).clone()
+        results.append((L, [x.clone() for x in prob.gradients()]))
+        # a second backward recomputes the segments the first one took from the forward: the same bits
         results.append((L, [x.clone() for x in prob.gradients()]))
     for L, g in results[1:]:
         assert torch.equal(L, results[0][0])
