@@ -4,12 +4,13 @@
 //
 // Per step and target row, the synaptic input keeps the reference CPU order (flyverse/brain.py:832-833, 679-692):
 // g = g*a_s, then the in-edges whose presynaptic cell fired n_delay steps ago, in ascending presynaptic order; the
-// LIF update is flyverse/brain.py:838-868 with the Shiu settings.  Spikes are bits in a ring of 64 slots (one bit per
-// cell per step).  Rows are independent for n_delay consecutive steps (their input comes from spikes at least
-// n_delay steps old), so the grid synchronises only once per window of <= n_delay steps: a wavefront takes a row,
-// folds its in-edges (lanes over edges: ballot of the fired bits, then the fired weights added in edge order) and
-// updates it for every step of the window with its state in registers.  Each window also clears the ring slots the
-// next window writes; those slots are older than anything the current window reads (64 > 2 * n_delay + window).
+// LIF update is flyverse/brain.py:838-868 with the Shiu settings.  Spikes are kept as one 64-bit history word per
+// cell: bit (t mod 64) says whether the cell fired at step t, so a word holds the cell's last 64 steps.  Rows are
+// independent for n_delay consecutive steps (their input comes from spikes at least n_delay steps old), so the grid
+// synchronises only once per window of <= n_delay steps; within a window a wavefront owns a block of 64 cells (see
+// packed_window).  A window rewrites the bits of its own steps in its cells' words and reads only bits of steps at
+// least n_delay older (64 >= 2 * n_delay + window keeps the two sets of bit positions apart), so a word that is read
+// while its owner rewrites it gives the reader the same bits either way.
 //
 // The Poisson uniform of a cell is Philox keyed by its instance's seed with counter (local cell index, step, 0, 0):
 // every instance draws exactly what it would draw run alone as a one-fly brain.
@@ -29,19 +30,22 @@ struct Packed {
     const int* ptr;          // (n + 1) post-major CSR over all packed cells
     const int* idx;          // (nnz) presynaptic cell (packed index)
     const float* w;          // (nnz)
-    const int* sched;        // (n) rows, longest first
+    const int* sched;        // (words) blocks of 64 cells, most in-edges first
     float *v, *g, *refrac, *rate, *counts;
     const float* drive;      // (n)
     const float* pois;       // (n) per-step Poisson probability
     const int* inst;         // (n) instance of each cell
     const int* local;        // (n) index of the cell within its instance
     const uint64_t* seeds;   // (instances) Philox keys
-    uint64_t* ring;          // (kSlots, words) fired bits
+    uint64_t* ring;          // (n) spike history words: bit (t mod 64) of word c = cell c fired at step t
     unsigned* bar_count;     // grid barrier arrivals
     unsigned* bar_gen;       // grid barrier generation
     int* error;              // (4) set when a barrier wait exceeds its bound: flag, arrivals seen, grid, window
     const float* P;          // (12) v_rest, v_reset, v_th, a_m, dt, t_ref, -, -, -, a_r, rate_gain, a_s
-    int n, words, steps, window, delay, t0;
+    const int* long_rows;    // (n_long) cells with more than 512 in-edges, most first
+    const int* long_id;      // (n) index of a long cell in long_rows, -1 for the others
+    float* long_g;           // (n_long, kMaxWindow) the long cells' synaptic chains of the current window
+    int n, words, steps, window, delay, t0, n_long;
     uint64_t spin_ticks;     // bound of one barrier wait, in wall-clock ticks
 };
 
@@ -77,30 +81,149 @@ __device__ __forceinline__ void grid_barrier(const Packed& a, unsigned nblocks, 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// One window: steps T0 .. T0 + K - 1 of every row.  Also clears the ring slots of the window that follows it.
+constexpr int kMaxWindow = 21;          // 2 * window + delay <= 64 with window <= delay
+constexpr int kShortEdges = 8 * kWave;  // cells with more in-edges are "long"
+
+// The window's view of the spike history: bit k of bits() = the cell fired at step T0 - delay + k (k < 32).
+struct Window {
+    int T0, K, s0;
+    float a_s;
+    __device__ Window(const Packed& a, int t0, int k) : T0(t0), K(k), s0((t0 - a.delay) & (kSlots - 1)), a_s(a.P[11]) {}
+    __device__ __forceinline__ uint32_t bits(uint64_t w) const {
+        return (uint32_t)(s0 ? (w >> s0) | (w << (kSlots - s0)) : w);
+    }
+};
+
+// The synaptic chain of one cell over the window, lanes over its in-edges [e0, e1) (at most NC * 64): g_k =
+// g_{k-1} * a_s, then the weights of the in-edges whose presynaptic cell fired at step T0 + k - delay, added in
+// edge order (ascending presynaptic index).  Each edge's weight and history word are read once for the whole window.
+// Returns g_k in lane k.
+template <int NC>
+__device__ __forceinline__ float g_chain_cached(const Packed& a, const Window& W, int e0, int e1, float acc, int lane) {
+    int pre[NC];
+    float wc[NC];
+    uint32_t hist[NC];
+    const int last = max(e1 - 1, e0);
+#pragma unroll
+    for (int u = 0; u < NC; ++u) {
+        const int e = e0 + u * kWave + lane;
+        pre[u] = e1 > e0 ? a.idx[min(e, last)] : 0;
+        wc[u] = e1 > e0 ? a.w[min(e, last)] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < NC; ++u) {
+        const bool val = e0 + u * kWave + lane < e1;
+        hist[u] = val ? W.bits(a.ring[pre[u]]) : 0u;
+    }
+    const int nch = (e1 - e0 + kWave - 1) / kWave;
+    float gk = 0.0f;
+    for (int k = 0; k < W.K; ++k) {
+        acc = acc * W.a_s;
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+            if (u < nch) {
+                uint64_t m = __builtin_amdgcn_ballot_w64((hist[u] >> k) & 1u);
+                while (m) {
+                    const int j = __builtin_ctzll(m);
+                    acc = acc + fh_readlane(wc[u], j);
+                    m &= m - 1;
+                }
+            }
+        }
+        gk = lane == k ? acc : gk;
+    }
+    return gk;
+}
+
+// The same chain for a cell of any length, streaming its in-edges (256 at a time) once per step.
+__device__ __forceinline__ float g_chain_stream(const Packed& a, const Window& W, int e0, int e1, float acc, int lane) {
+    float gk = 0.0f;
+    for (int k = 0; k < W.K; ++k) {
+        acc = acc * W.a_s;
+        for (int base = e0; base < e1; base += 4 * kWave) {
+            int pre[4];
+            bool act[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = base + u * kWave + lane;
+                pre[u] = e < e1 ? a.idx[e] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) act[u] = pre[u] >= 0 && ((W.bits(a.ring[max(pre[u], 0)]) >> k) & 1u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (base + u * kWave >= e1) break;
+                const float c = act[u] ? a.w[base + u * kWave + lane] : 0.0f;
+                uint64_t m = __builtin_amdgcn_ballot_w64(act[u]);
+                while (m) {
+                    const int j = __builtin_ctzll(m);
+                    acc = acc + fh_readlane(c, j);
+                    m &= m - 1;
+                }
+            }
+        }
+        gk = lane == k ? acc : gk;
+    }
+    return gk;
+}
+
+// One window: steps T0 .. T0 + K - 1 of every cell.  A wavefront takes a block of 64 consecutive cells (blocks in
+// the order of `sched`) in two phases:
+//  A. for each cell of the block in turn, lanes over its in-edges: the synaptic chain of the window (g_chain_*); it
+//     needs nothing of the membrane, so it runs ahead of the LIF update.  g_k of the cell goes to LDS.  With
+//     LONG_BUF the chains of the long cells (more than 512 in-edges) were computed before by k_packed_long.
+//  B. lanes over the 64 cells: the LIF update of every step of the window with g_k from LDS; each lane then
+//     rewrites the window's bits of its own cell's history word with one plain store (the block's cells are this
+//     wavefront's alone, so no atomics are needed).
+template <bool LONG_BUF>
 __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
+    __shared__ float gbuf[4][kWave * kMaxWindow];
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = gridDim.x * 4;
-    const int gthread = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nthreads = gridDim.x * blockDim.x;
+    float* const gb = gbuf[threadIdx.x >> 6];
     const float v_rest = a.P[0], v_reset = a.P[1], v_th = a.P[2], am = a.P[3], dt = a.P[4], t_ref = a.P[5];
-    const float ar = a.P[9], rg = a.P[10], a_s = a.P[11];
-    // clear the slots the next window writes (steps T0 + K .. T0 + K + window - 1)
-    for (int64_t q = gthread; q < (int64_t)a.window * a.words; q += nthreads) {
-        const int s = (T0 + K + (int)(q / a.words)) & (kSlots - 1);
-        a.ring[(int64_t)s * a.words + q % a.words] = 0ull;
-    }
-    for (int si = wave; si < a.n; si += nwaves) {
-        const int r = a.sched[si];
-        const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
-        float v = a.v[r], g = a.g[r], ref = a.refrac[r], rate = a.rate[r], cnt = a.counts[r];
-        const float drive = a.drive[r], p = a.pois[r];
+    const float ar = a.P[9], rg = a.P[10];
+    const Window W(a, T0, K);
+    for (int bi = wave; bi < a.words; bi += nwaves) {
+        const int wd = a.sched[bi];
+        const int c0 = wd * 64;
+        const int nc = min(64, a.n - c0);
+        // the block's row bounds and synaptic states, one lane per cell
+        const int pl0 = a.ptr[c0 + min(lane, nc)], pl1 = a.ptr[c0 + min(lane + 1, nc)];
+        const float gl = lane < nc ? a.g[c0 + lane] : 0.0f;
+        // ---------------------------------------------------------------- phase A
+        for (int i = 0; i < nc; ++i) {
+            const int e0 = __builtin_amdgcn_readlane(pl0, i), e1 = __builtin_amdgcn_readlane(pl1, i);
+            const float g0 = fh_readlane(gl, i);
+            float gk;
+            if (e1 - e0 <= 4 * kWave) {
+                gk = g_chain_cached<4>(a, W, e0, e1, g0, lane);
+            } else if (e1 - e0 <= kShortEdges) {
+                gk = g_chain_cached<8>(a, W, e0, e1, g0, lane);
+            } else {
+                const int li = LONG_BUF && a.n_long > 0 ? a.long_id[c0 + i] : -1;
+                if (li >= 0)
+                    gk = lane < K ? a.long_g[(int64_t)li * kMaxWindow + lane] : 0.0f;
+                else
+                    gk = g_chain_stream(a, W, e0, e1, g0, lane);
+            }
+            if (lane < K) gb[i * kMaxWindow + lane] = gk;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // ---------------------------------------------------------------- phase B
+        // LIF update of flyverse/brain.py:838-868 (Shiu settings), lane = cell c0 + lane
+        const bool ok = lane < nc;
+        const int r = c0 + (ok ? lane : 0);
+        float v = a.v[r], ref = a.refrac[r], rate = a.rate[r], cnt = a.counts[r], g = gl;
+        const float drive = a.drive[r], p = ok ? a.pois[r] : 0.0f;
         const uint64_t seed = a.seeds[a.inst[r]];
         const uint32_t loc = (uint32_t)a.local[r];
-        // LIF update of step k (uniform across the wavefront: every lane computes the same row)
-        auto lif_step = [&](int k, float acc) {
+        const uint64_t old_hist = a.ring[r];
+        uint64_t mine = 0, fired_bits = 0;    // the window's bit positions, and those of the steps the cell fired
+        for (int k = 0; k < K; ++k) {
             const int t = T0 + k;
+            const float acc = gb[lane * kMaxWindow + k];
             float target = v_rest + acc;
             target = target + drive;
             float dv = v - target;
@@ -117,107 +240,50 @@ __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
             float rr = rate * ar;
             float rs = spike * rg;
             g = acc; v = vn; ref = rn; rate = rr + rs; cnt = cnt + spike;
-            if (fired && lane == 0)
-                atomicOr((unsigned long long*)(a.ring + (int64_t)(t & (kSlots - 1)) * a.words + (r >> 6)),
-                         1ull << (r & 63));
-        };
-        auto slot_of = [&](int k) {
-            return a.ring + (int64_t)((T0 + k - a.delay) & (kSlots - 1)) * a.words;
-        };
-        // at most NC * 64 in-edges: presynaptic indices and weights stay in registers for the window, and the bit
-        // words of step k + 1 are gathered before step k is folded (two buffers alternating, no copies)
-        auto cached = [&](auto nc_tag) {
-            constexpr int NC = decltype(nc_tag)::value;
-            int pre[NC];
-            float wc[NC];
-            bool val[NC];
-            const int last = max(e1 - 1, e0);
-#pragma unroll
-            for (int u = 0; u < NC; ++u) {
-                const int e = e0 + u * kWave + lane;
-                val[u] = e < e1;
-                pre[u] = e1 > e0 ? a.idx[min(e, last)] : 0;
-                wc[u] = e1 > e0 ? a.w[min(e, last)] : 0.0f;
-            }
-            const int nch = (e1 - e0 + kWave - 1) / kWave;
-            uint64_t wa[NC], wb[NC];
-            auto look = [&](int k, uint64_t* wd) {
-                const uint64_t* slot = slot_of(k);
-#pragma unroll
-                for (int u = 0; u < NC; ++u) wd[u] = slot[pre[u] >> 6];
-            };
-            auto step = [&](int k, const uint64_t* wd) {
-                float acc = g * a_s;
-#pragma unroll
-                for (int u = 0; u < NC; ++u) {
-                    if (u < nch) {
-                        const bool act = val[u] && ((wd[u] >> (pre[u] & 63)) & 1ull);
-                        uint64_t m = __builtin_amdgcn_ballot_w64(act);
-                        while (m) {
-                            const int j = __builtin_ctzll(m);
-                            acc = acc + fh_readlane(wc[u], j);
-                            m &= m - 1;
-                        }
-                    }
-                }
-                lif_step(k, acc);
-            };
-            look(0, wa);
-            for (int k = 0; k < K; k += 2) {
-                look(min(k + 1, K - 1), wb);
-                step(k, wa);
-                look(min(k + 2, K - 1), wa);
-                if (k + 1 < K) step(k + 1, wb);
-            }
-        };
-        if (e1 - e0 <= 4 * kWave) {
-            cached(std::integral_constant<int, 4>{});
-        } else if (e1 - e0 <= 8 * kWave) {
-            cached(std::integral_constant<int, 8>{});
-        } else {
-            for (int k = 0; k < K; ++k) {
-                const uint64_t* slot = slot_of(k);
-                float acc = g * a_s;
-                for (int base = e0; base < e1; base += 4 * kWave) {
-                    int pre[4];
-                    bool act[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int e = base + u * kWave + lane;
-                        pre[u] = e < e1 ? a.idx[e] : -1;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        act[u] = pre[u] >= 0 && ((slot[pre[u] >> 6] >> (pre[u] & 63)) & 1ull);
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (base + u * kWave >= e1) break;
-                        const float c = act[u] ? a.w[base + u * kWave + lane] : 0.0f;
-                        uint64_t m = __builtin_amdgcn_ballot_w64(act[u]);
-                        while (m) {
-                            const int j = __builtin_ctzll(m);
-                            acc = acc + fh_readlane(c, j);
-                            m &= m - 1;
-                        }
-                    }
-                }
-                lif_step(k, acc);
-            }
+            const uint64_t bit = 1ull << (t & (kSlots - 1));
+            mine |= bit;
+            if (fired) fired_bits |= bit;
         }
-        if (lane == 0) { a.v[r] = v; a.g[r] = g; a.refrac[r] = ref; a.rate[r] = rate; a.counts[r] = cnt; }
+        if (ok) {
+            a.v[r] = v; a.g[r] = g; a.refrac[r] = ref; a.rate[r] = rate; a.counts[r] = cnt;
+            a.ring[r] = (old_hist & ~mine) | fired_bits;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
 __global__ __launch_bounds__(256) void k_packed_run(Packed a) {
     for (int T0 = a.t0; T0 < a.t0 + a.steps; T0 += a.window) {
-        packed_window(a, T0, min(a.window, a.t0 + a.steps - T0));
+        packed_window<false>(a, T0, min(a.window, a.t0 + a.steps - T0));
         grid_barrier(a, gridDim.x, T0);
     }
 }
 
 // One window per launch; the window's first step is *t_base + off.
 __global__ __launch_bounds__(256) void k_packed_window(Packed a, const int* t_base, int off, int K) {
-    packed_window(a, *t_base + off, K);
+    packed_window<true>(a, *t_base + off, K);
+}
+
+// The synaptic chains of the long cells for the window that k_packed_window runs next, one wavefront per cell
+// (long_rows, most in-edges first): edges and history words held in registers up to 2048 in-edges, streamed per
+// step beyond.  g_k of long cell i goes to long_g[i * kMaxWindow + k].
+__global__ __launch_bounds__(256) void k_packed_long(Packed a, const int* t_base, int off, int K) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const Window W(a, *t_base + off, K);
+    for (int i = wave; i < a.n_long; i += gridDim.x * 4) {
+        const int r = a.long_rows[i];
+        const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
+        const float g0 = a.g[r];
+        float gk;
+        if (e1 - e0 <= 16 * kWave)
+            gk = g_chain_cached<16>(a, W, e0, e1, g0, lane);
+        else if (e1 - e0 <= 32 * kWave)
+            gk = g_chain_cached<32>(a, W, e0, e1, g0, lane);
+        else
+            gk = g_chain_stream(a, W, e0, e1, g0, lane);
+        if (lane < K) a.long_g[(int64_t)i * kMaxWindow + lane] = gk;
+    }
 }
 
 __global__ void k_advance(int* t_base, int steps) {
@@ -231,6 +297,7 @@ static void fill_packed(Packed& a, void** p) {
     a.local = (const int*)p[12]; a.seeds = (const uint64_t*)p[13]; a.ring = (uint64_t*)p[14];
     a.bar_count = (unsigned*)p[15]; a.bar_gen = (unsigned*)p[15] + 1; a.error = (int*)p[16];
     a.P = (const float*)p[17];
+    a.long_rows = nullptr; a.long_id = nullptr; a.long_g = nullptr; a.n_long = 0;
 }
 
 // Workgroups of k_packed_run that are resident together: one per CU fewer than the occupancy query reports (on
@@ -239,18 +306,18 @@ static void fill_packed(Packed& a, void** p) {
 static int resident_grid(int per_cu, int cus) { return (per_cu > 1 ? per_cu - 1 : per_cu) * cus; }
 
 static bool packed_shape_ok(const Packed& a) {
-    return a.n >= 0 && a.steps >= 0 && a.window >= 1 && a.window <= a.delay && 2 * a.delay + a.window <= kSlots &&
-           2 * a.window + a.delay <= kSlots && a.t0 >= 0;
+    return a.n >= 0 && a.steps >= 0 && a.window >= 1 && a.window <= a.delay && a.window <= kMaxWindow &&
+           2 * a.delay + a.window <= kSlots && 2 * a.window + a.delay <= kSlots && a.t0 >= 0;
 }
 
 }  // namespace
 
 // Run `steps` LIF steps (from global step t0) of the packed graph in one cooperative launch.
-// p = [ptr, idx, w, sched, v, g, refrac, rate, counts, drive, pois, inst, local, seeds, ring (64, words),
+// p = [ptr, idx, w, sched (ceil(n/64) blocks of 64 cells, in visiting order), v, g, refrac, rate, counts, drive,
+//      pois, inst, local, seeds, ring (n spike history words: bit t mod 64 of word c = cell c fired at step t),
 //      bar (2 uint32: count, generation), error (4 int32), P (12)];
 // dims = [n, steps, window, delay, grid, t0, barrier bound in ms].  Requires window <= delay and
-// 2 * delay + window <= 64; a run that continues an earlier one (t0 > 0) must use the same window, whose last window
-// cleared the next slots.
+// 2 * delay + window <= 64 (so window <= 21).
 FH_EXPORT int launch_packed_run(void** p, const int* dims, void* stream) {
     Packed a;
     fill_packed(a, p);
@@ -278,18 +345,32 @@ FH_EXPORT int launch_packed_run(void** p, const int* dims, void* stream) {
 }
 
 // One window of K steps starting at step *t_base + off (t_base: int32 on the device, read when the kernel runs, so a
-// captured graph of windows replays at advancing steps).  p as launch_packed_run (bar and error unused);
-// dims = [n, K, window, delay, grid, off].  The kernel boundary orders the window against the next one.
+// captured graph of windows replays at advancing steps).  p as launch_packed_run (bar and error unused) followed by
+// [long_rows (n_long), long_id (n), long_g (n_long, 21) scratch]: the cells with more than 512 in-edges, which a
+// first kernel steps one wavefront per cell (long_id = -1 elsewhere; n_long = 0 streams them inside the window
+// kernel instead); dims = [n, K, window, delay, grid, off, n_long].  The kernel boundaries order the window against
+// the next one.
 FH_EXPORT int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream) {
     Packed a;
     fill_packed(a, p);
     a.n = dims[0]; a.steps = dims[1]; a.window = dims[2]; a.delay = dims[3];
     const int grid = dims[4], off = dims[5];
+    a.n_long = dims[6];
     a.t0 = 0;
     a.words = (a.n + 63) / 64;
     a.spin_ticks = 0;
-    if (!packed_shape_ok(a) || a.steps > a.window || grid <= 0 || off < 0 || t_base == nullptr) return FH_ERR_ARGS;
+    if (!packed_shape_ok(a) || a.steps > a.window || grid <= 0 || off < 0 || t_base == nullptr || a.n_long < 0 ||
+        a.n_long > a.n)
+        return FH_ERR_ARGS;
+    if (a.n_long > 0) {
+        a.long_rows = (const int*)p[18]; a.long_id = (const int*)p[19]; a.long_g = (float*)p[20];
+        if (!a.long_rows || !a.long_id || !a.long_g) return FH_ERR_ARGS;
+    }
     if (a.n == 0 || a.steps == 0) return 0;
+    if (a.n_long > 0) {
+        const int lgrid = min(grid, (a.n_long + 3) / 4);
+        hipLaunchKernelGGL(k_packed_long, dim3(lgrid), dim3(256), 0, (hipStream_t)stream, a, t_base, off, a.steps);
+    }
     hipLaunchKernelGGL(k_packed_window, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, t_base, off, a.steps);
     FH_LAUNCH_CHECK();
 }

@@ -94,8 +94,10 @@ def tune_packed(pb, steps: int = 180, windows=None, grids=None, use_cache=True, 
         return table[key] | {"cached": True}
     cus = torch.cuda.get_device_properties(pb.device).multi_processor_count
     windows = windows or sorted({w for w in (pb.n_delay, (pb.n_delay + 1) // 2) if w >= 1})
-    cap = pb.max_grid if pb.mode == "coop" else cus * 32
-    grids = grids or sorted({min(cap, cus * m) for m in (4, 8, 12, 16, 24)})
+    # a wavefront takes one block of 64 cells at a time: grids past one block per wavefront add nothing (grids stay
+    # multiples of 8, the XCD count the block schedule is dealt over)
+    cap = pb.max_grid if pb.mode == "coop" else max(8, min(cus * 64, -(-pb.words // 32) * 8))
+    grids = grids or sorted({min(cap, cus * m) for m in (4, 8, 16, 24, 32, 48, 64)})
     trials = []
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for w, g in itertools.product(windows, grids):

@@ -155,15 +155,18 @@ int launch_reset_lanes(void** p, uint64_t mask, uint64_t live_set, float v_rest,
 /* ------------------------------------------------------------------ packed uneven subgraphs
  * Many LIF graphs packed block-diagonally into one CSR with offset tables (instance / local index per cell, one
  * Philox key per instance).  The reference would run one Brain per Connectome.subset (flyverse/connectome.py:469-481).
- * p = [ptr, idx, w, sched, v, g, refrac, rate, counts, drive, pois, inst, local, seeds uint64, ring uint64 (64,
- * ceil(n/64)), bar (2 uint32), error (4 int32), P(12)]. */
+ * p = [ptr, idx, w, sched (ceil(n/64) blocks of 64 cells, visiting order), v, g, refrac, rate, counts, drive, pois,
+ * inst, local, seeds uint64, ring uint64 (n spike history words: bit t % 64 of word c = cell c fired at step t),
+ * bar (2 uint32), error (4 int32), P(12)]. */
 
 /* `steps` steps from step t0 in one cooperative launch; dims = [n, steps, window, delay, grid, t0, barrier bound ms];
  * window <= delay, 2*delay + window <= 64, grid <= packed_max_grid. */
 int launch_packed_run(void** p, const int* dims, void* stream);
 
 /* One window of K <= window steps from step *t_base + off (t_base: device int32, so a captured graph of windows
- * replays at advancing steps); dims = [n, K, window, delay, grid, off]. */
+ * replays at advancing steps); p continues with [long_rows (n_long: cells with more than 512 in-edges), long_id (n:
+ * index in long_rows or -1), long_g (n_long * 21 floats, scratch)]; dims = [n, K, window, delay, grid, off,
+ * n_long] (n_long = 0: no long-cell kernel, those cells are streamed in the window kernel). */
 int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream);
 
 /* *t_base += steps on the stream. */
