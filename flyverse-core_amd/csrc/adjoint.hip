@@ -296,35 +296,23 @@ __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
     }
 }
 
-// Backward, part 3 (per edge of W): eg[e] += sum over movies of ly[i, b] * dr_t[col_e, b], lanes over edges of a row,
-// the 32 movies in order.
-__global__ __launch_bounds__(256) void k_adj_back_edges(const int* __restrict__ ptr, const int* __restrict__ idx,
+// Backward, part 3 (per edge of W): eg[e] += sum over movies of ly[row_e, b] * dr_t[col_e, b], one thread per edge
+// (all edges in one flat grid, so short rows leave no lanes idle and no row-by-row load chain), the 32 movies in order.
+__global__ __launch_bounds__(256) void k_adj_back_edges(const int* __restrict__ row_of, const int* __restrict__ idx,
                                                         const float* __restrict__ ly, const float* __restrict__ dr,
-                                                        float* __restrict__ eg, int rows) {
-    const int lane = threadIdx.x & 63;
-    for (int r = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); r < rows; r += gridDim.x * 4) {
-        const int e0 = ptr[r], e1 = ptr[r + 1];
-        if (e0 == e1) continue;
-        float lyr[NL];
-        const float4* ly4 = (const float4*)(ly + (size_t)r * NL);
+                                                        float* __restrict__ eg, int64_t nnz) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const float4* l4 = (const float4*)(ly + (size_t)row_of[e] * NL);
+    const float4* d4 = (const float4*)(dr + (size_t)idx[e] * NL);
+    float s = 0.0f;
 #pragma unroll
-        for (int k = 0; k < NL / 4; ++k) {
-            float4 t = ly4[k];
-            lyr[4 * k] = t.x; lyr[4 * k + 1] = t.y; lyr[4 * k + 2] = t.z; lyr[4 * k + 3] = t.w;
-        }
-        for (int e = e0 + lane; e < e1; e += 64) {
-            const float4* d4 = (const float4*)(dr + (size_t)idx[e] * NL);
-            float s = 0.0f;
-#pragma unroll
-            for (int k = 0; k < NL / 4; ++k) {
-                float4 t = d4[k];
-                float p0 = lyr[4 * k] * t.x, p1 = lyr[4 * k + 1] * t.y, p2 = lyr[4 * k + 2] * t.z,
-                      p3 = lyr[4 * k + 3] * t.w;
-                s = s + p0; s = s + p1; s = s + p2; s = s + p3;
-            }
-            eg[e] = eg[e] + s;
-        }
+    for (int k = 0; k < NL / 4; ++k) {
+        const float4 a = l4[k], t = d4[k];
+        float p0 = a.x * t.x, p1 = a.y * t.y, p2 = a.z * t.z, p3 = a.w * t.w;
+        s = s + p0; s = s + p1; s = s + p2; s = s + p3;
     }
+    eg[e] = eg[e] + s;
 }
 
 // out[k] = scale_k * sum over the segment [seg[k], seg[k+1]) of vals[perm[m]] * (mult ? mult[perm[m]] : 1):
@@ -438,12 +426,12 @@ FH_EXPORT int launch_adj_back_rows(void** p, int rows, const float* f, int grid,
     FH_LAUNCH_CHECK();
 }
 
-// p = [ptr, idx, ly, dr, eg]
-FH_EXPORT int launch_adj_back_edges(void** p, int rows, int grid, void* stream) {
-    if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
-    if (!rows) return 0;
-    hipLaunchKernelGGL(k_adj_back_edges, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
-                       (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], rows);
+// p = [row_of (nnz: the row of each edge), idx (nnz), ly (n, 32), dr (n, 32), eg (nnz)]
+FH_EXPORT int launch_adj_back_edges(void** p, int64_t nnz, void* stream) {
+    if (nnz < 0) return FH_ERR_ARGS;
+    if (!nnz) return 0;
+    hipLaunchKernelGGL(k_adj_back_edges, dim3(nblk(nnz)), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
+                       (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], nnz);
     FH_LAUNCH_CHECK();
 }
 

@@ -40,7 +40,7 @@ SIGS = {
     "launch_adj_forward": [_P, _I, _FP, _I, _S],
     "launch_adj_back_ly": [_P, _I, _F, _S],
     "launch_adj_back_rows": [_P, _I, _FP, _I, _S],
-    "launch_adj_back_edges": [_P, _I, _I, _S],
+    "launch_adj_back_edges": [_P, ct.c_int64, _S],
     "launch_segment_sum": [_P, _I, _S],
     "launch_loss_total": [_P, _L, _F, _S],
 }
@@ -102,7 +102,7 @@ class OpticAdjoint:
             raise ValueError("checkpoint must be >= 1")
         t = lambda a, dt=None: torch.as_tensor(np.ascontiguousarray(a), device=dev, dtype=dt)
         rows = np.repeat(np.arange(n), np.diff(W0.indptr))
-        self.ptr, self.idx = t(W0.indptr.astype(np.int32)), t(W0.indices.astype(np.int32))
+        self.row_of, self.idx = t(rows.astype(np.int32)), t(W0.indices.astype(np.int32))
         self.w0 = t(W0.data)
         bins = (ut[rows] * K + ut[W0.indices]).astype(np.int32)
         self.sched = t(np.argsort(-np.diff(W0.indptr), kind="stable").astype(np.int32))
@@ -259,7 +259,7 @@ class OpticAdjoint:
                 self._call("launch_adj_back_ly", [self.lv, self.S_v[k], self.S_inp[k], self.a, self.ly, self.ga], n,
                            float(self.c.gain_rr))
                 self._back_rows(k, t, f_back)
-                self._call("launch_adj_back_edges", [self.ptr, self.idx, self.ly, self.S_dr[k], self.eg], n, self.grid)
+                self._call("launch_adj_back_edges", [self.row_of, self.idx, self.ly, self.S_dr[k], self.eg], self.nnz)
         tau, bias, gain = self._params
         K = self.K
         ga_t, gb_t, gg = (torch.empty(K, device=self.device), torch.empty(K, device=self.device),
