@@ -45,7 +45,7 @@ struct Packed {
     const int* long_rows;    // (n_long) cells with more than 512 in-edges, most first
     const int* long_id;      // (n) index of a long cell in long_rows, -1 for the others
     float* long_g;           // (n_long, kMaxWindow) the long cells' synaptic chains of the current window
-    int n, words, steps, window, delay, t0, n_long;
+    int n, words, steps, window, delay, t0, n_long, n_vlong;
     uint64_t spin_ticks;     // bound of one barrier wait, in wall-clock ticks
 };
 
@@ -83,6 +83,7 @@ __device__ __forceinline__ void grid_barrier(const Packed& a, unsigned nblocks, 
 
 constexpr int kMaxWindow = 21;          // 2 * window + delay <= 64 with window <= delay
 constexpr int kShortEdges = 8 * kWave;  // cells with more in-edges are "long"
+constexpr int kVlongChunks = 36;        // chunks per wavefront in k_packed_vlong (cells of up to 9216 in-edges)
 
 // The window's view of the spike history: bit k of bits() = the cell fired at step T0 - delay + k (k < 32).
 struct Window {
@@ -97,12 +98,8 @@ struct Window {
 // The synaptic chain of one cell over the window, lanes over its in-edges [e0, e1) (at most NC * 64): g_k =
 // g_{k-1} * a_s, then the weights of the in-edges whose presynaptic cell fired at step T0 + k - delay, added in
 // edge order (ascending presynaptic index).  Each edge's weight and history word are read once for the whole window.
-// Returns g_k in lane k.
 template <int NC>
-__device__ __forceinline__ float g_chain_cached(const Packed& a, const Window& W, int e0, int e1, float acc, int lane) {
-    int pre[NC];
-    float wc[NC];
-    uint32_t hist[NC];
+__device__ __forceinline__ void load_edges(const Packed& a, int e0, int e1, int lane, int* pre, float* wc) {
     const int last = max(e1 - 1, e0);
 #pragma unroll
     for (int u = 0; u < NC; ++u) {
@@ -110,11 +107,13 @@ __device__ __forceinline__ float g_chain_cached(const Packed& a, const Window& W
         pre[u] = e1 > e0 ? a.idx[min(e, last)] : 0;
         wc[u] = e1 > e0 ? a.w[min(e, last)] : 0.0f;
     }
-#pragma unroll
-    for (int u = 0; u < NC; ++u) {
-        const bool val = e0 + u * kWave + lane < e1;
-        hist[u] = val ? W.bits(a.ring[pre[u]]) : 0u;
-    }
+}
+
+// the steps from the window's bits of each edge (bit k: the presynaptic cell fired at step T0 - delay + k);
+// returns g_k in lane k
+template <int NC>
+__device__ __forceinline__ float chain_steps_bits(const Window& W, int e0, int e1, float acc, int lane,
+                                                  const float* wc, const uint32_t* hist) {
     const int nch = (e1 - e0 + kWave - 1) / kWave;
     float gk = 0.0f;
     for (int k = 0; k < W.K; ++k) {
@@ -133,6 +132,25 @@ __device__ __forceinline__ float g_chain_cached(const Packed& a, const Window& W
         gk = lane == k ? acc : gk;
     }
     return gk;
+}
+
+template <int NC>
+__device__ __forceinline__ float g_chain_cached(const Packed& a, const Window& W, int e0, int e1, float acc, int lane) {
+    int pre[NC];
+    float wc[NC];
+    load_edges<NC>(a, e0, e1, lane, pre, wc);
+    // history words 8 chunks at a time, folded to the window's bits at once (few registers for long cells)
+    uint32_t hist[NC];
+#pragma unroll
+    for (int u0 = 0; u0 < NC; u0 += 8) {
+        uint64_t raw[8];
+#pragma unroll
+        for (int v = 0; v < 8 && u0 + v < NC; ++v) raw[v] = a.ring[pre[u0 + v]];
+#pragma unroll
+        for (int v = 0; v < 8 && u0 + v < NC; ++v)
+            hist[u0 + v] = e0 + (u0 + v) * kWave + lane < e1 ? W.bits(raw[v]) : 0u;
+    }
+    return chain_steps_bits<NC>(W, e0, e1, acc, lane, wc, hist);
 }
 
 // The same chain for a cell of any length, streaming its in-edges (256 at a time) once per step.
@@ -271,7 +289,7 @@ __global__ __launch_bounds__(256) void k_packed_long(Packed a, const int* t_base
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const Window W(a, *t_base + off, K);
-    for (int i = wave; i < a.n_long; i += gridDim.x * 4) {
+    for (int i = a.n_vlong + wave; i < a.n_long; i += gridDim.x * 4) {
         const int r = a.long_rows[i];
         const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
         const float g0 = a.g[r];
@@ -286,6 +304,73 @@ __global__ __launch_bounds__(256) void k_packed_long(Packed a, const int* t_base
     }
 }
 
+// The synaptic chains of the very long cells (the first n_vlong of long_rows, more than 2048 in-edges), one
+// workgroup per cell: wavefront w holds chunks [w * NCW, (w + 1) * NCW) of the cell's in-edges in registers, and for
+// each step the running sum passes from wavefront to wavefront in edge order through LDS (one barrier per segment).
+// Cells longer than 4 * NCW chunks are streamed by wavefront 0.
+template <int NCW>
+__global__ __launch_bounds__(256) void k_packed_vlong(Packed a, const int* t_base, int off, int K) {
+    __shared__ float carry;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const Window W(a, *t_base + off, K);
+    for (int i = blockIdx.x; i < a.n_vlong; i += gridDim.x) {
+        const int r = a.long_rows[i];
+        const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
+        const int nch = (e1 - e0 + kWave - 1) / kWave;
+        const float g0 = a.g[r];
+        if (nch > 4 * NCW) {
+            if (wv == 0) {
+                const float gk = g_chain_stream(a, W, e0, e1, g0, lane);
+                if (lane < K) a.long_g[(int64_t)i * kMaxWindow + lane] = gk;
+            }
+            __syncthreads();
+            continue;
+        }
+        const int mych = min(max(nch - wv * NCW, 0), NCW);
+        int pre[NCW];
+        float wc[NCW];
+        uint32_t hist[NCW];
+#pragma unroll
+        for (int u = 0; u < NCW; ++u) {
+            const int e = e0 + (wv * NCW + u) * kWave + lane;
+            const bool val = u < mych && e < e1;
+            pre[u] = val ? a.idx[e] : 0;
+            wc[u] = val ? a.w[e] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < NCW; ++u) {
+            const bool val = u < mych && e0 + (wv * NCW + u) * kWave + lane < e1;
+            hist[u] = val ? W.bits(a.ring[pre[u]]) : 0u;
+        }
+        float acc = g0, gk = 0.0f;
+        for (int k = 0; k < K; ++k) {
+            for (int seg = 0; seg < 4; ++seg) {
+                if (wv == seg) {
+                    float x = seg == 0 ? acc * W.a_s : carry;
+#pragma unroll
+                    for (int u = 0; u < NCW; ++u) {
+                        if (u < mych) {
+                            uint64_t m = __builtin_amdgcn_ballot_w64((hist[u] >> k) & 1u);
+                            while (m) {
+                                const int j = __builtin_ctzll(m);
+                                x = x + fh_readlane(wc[u], j);
+                                m &= m - 1;
+                            }
+                        }
+                    }
+                    if (lane == 0) carry = x;
+                }
+                __syncthreads();
+            }
+            acc = carry;
+            gk = lane == k ? acc : gk;
+            __syncthreads();
+        }
+        if (wv == 0 && lane < K) a.long_g[(int64_t)i * kMaxWindow + lane] = gk;
+    }
+}
+
 __global__ void k_advance(int* t_base, int steps) {
     if (threadIdx.x == 0) *t_base += steps;
 }
@@ -297,7 +382,7 @@ static void fill_packed(Packed& a, void** p) {
     a.local = (const int*)p[12]; a.seeds = (const uint64_t*)p[13]; a.ring = (uint64_t*)p[14];
     a.bar_count = (unsigned*)p[15]; a.bar_gen = (unsigned*)p[15] + 1; a.error = (int*)p[16];
     a.P = (const float*)p[17];
-    a.long_rows = nullptr; a.long_id = nullptr; a.long_g = nullptr; a.n_long = 0;
+    a.long_rows = nullptr; a.long_id = nullptr; a.long_g = nullptr; a.n_long = 0; a.n_vlong = 0;
 }
 
 // Workgroups of k_packed_run that are resident together: one per CU fewer than the occupancy query reports (on
@@ -346,29 +431,33 @@ FH_EXPORT int launch_packed_run(void** p, const int* dims, void* stream) {
 
 // One window of K steps starting at step *t_base + off (t_base: int32 on the device, read when the kernel runs, so a
 // captured graph of windows replays at advancing steps).  p as launch_packed_run (bar and error unused) followed by
-// [long_rows (n_long), long_id (n), long_g (n_long, 21) scratch]: the cells with more than 512 in-edges, which a
-// first kernel steps one wavefront per cell (long_id = -1 elsewhere; n_long = 0 streams them inside the window
-// kernel instead); dims = [n, K, window, delay, grid, off, n_long].  The kernel boundaries order the window against
-// the next one.
+// [long_rows (n_long), long_id (n), long_g (n_long, 21) scratch]: the cells with more than 512 in-edges, most first,
+// whose synaptic chains first kernels compute (the first n_vlong, more than 2048 in-edges, one workgroup per cell;
+// the rest one wavefront per cell; long_id = -1 elsewhere; n_long = 0 streams them inside the window kernel instead);
+// dims = [n, K, window, delay, grid, off, n_long, n_vlong].  The kernel boundaries order the window against the next.
 FH_EXPORT int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream) {
     Packed a;
     fill_packed(a, p);
     a.n = dims[0]; a.steps = dims[1]; a.window = dims[2]; a.delay = dims[3];
     const int grid = dims[4], off = dims[5];
     a.n_long = dims[6];
+    a.n_vlong = dims[7];
     a.t0 = 0;
     a.words = (a.n + 63) / 64;
     a.spin_ticks = 0;
     if (!packed_shape_ok(a) || a.steps > a.window || grid <= 0 || off < 0 || t_base == nullptr || a.n_long < 0 ||
-        a.n_long > a.n)
+        a.n_long > a.n || a.n_vlong < 0 || a.n_vlong > a.n_long)
         return FH_ERR_ARGS;
     if (a.n_long > 0) {
         a.long_rows = (const int*)p[18]; a.long_id = (const int*)p[19]; a.long_g = (float*)p[20];
         if (!a.long_rows || !a.long_id || !a.long_g) return FH_ERR_ARGS;
     }
     if (a.n == 0 || a.steps == 0) return 0;
-    if (a.n_long > 0) {
-        const int lgrid = min(grid, (a.n_long + 3) / 4);
+    if (a.n_vlong > 0)
+        hipLaunchKernelGGL(k_packed_vlong<kVlongChunks>, dim3(min(a.n_vlong, 65535)), dim3(256), 0,
+                           (hipStream_t)stream, a, t_base, off, a.steps);
+    if (a.n_long > a.n_vlong) {
+        const int lgrid = min(grid, (a.n_long - a.n_vlong + 3) / 4);
         hipLaunchKernelGGL(k_packed_long, dim3(lgrid), dim3(256), 0, (hipStream_t)stream, a, t_base, off, a.steps);
     }
     hipLaunchKernelGGL(k_packed_window, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, t_base, off, a.steps);

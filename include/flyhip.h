@@ -164,9 +164,10 @@ int launch_reset_lanes(void** p, uint64_t mask, uint64_t live_set, float v_rest,
 int launch_packed_run(void** p, const int* dims, void* stream);
 
 /* One window of K <= window steps from step *t_base + off (t_base: device int32, so a captured graph of windows
- * replays at advancing steps); p continues with [long_rows (n_long: cells with more than 512 in-edges), long_id (n:
- * index in long_rows or -1), long_g (n_long * 21 floats, scratch)]; dims = [n, K, window, delay, grid, off,
- * n_long] (n_long = 0: no long-cell kernel, those cells are streamed in the window kernel). */
+ * replays at advancing steps); p continues with [long_rows (n_long: cells with more than 512 in-edges, most first),
+ * long_id (n: index in long_rows or -1), long_g (n_long * 21 floats, scratch)]; dims = [n, K, window, delay, grid,
+ * off, n_long, n_vlong] (the first n_vlong long cells have more than 2048 in-edges; n_long = 0: no long-cell kernels,
+ * those cells are streamed in the window kernel). */
 int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream);
 
 /* *t_base += steps on the stream. */
