@@ -16,7 +16,7 @@ import scipy.sparse as sp
 from flyhip import hip, synth
 from flyhip.brain import Brain, LIFParams, shaped_weights
 from flyhip.connectome import Connectome
-from flyhip.packed import PackedBrains, pack_instances
+from flyhip.packed import PackedBrains, block_schedule, pack_instances
 
 ROOT = Path(__file__).resolve().parents[1]
 
@@ -124,6 +124,25 @@ def test_pack_instances_is_the_block_diagonal():
     ref.sort_indices()
     assert np.array_equal(indptr, ref.indptr) and np.array_equal(indices, ref.indices)
     assert np.array_equal(data, ref.data)
+
+
+def test_block_schedule_visits_every_block_once_and_deals_instances_to_xcds():
+    rng = np.random.default_rng(4)
+    sizes = rng.integers(40, 900, 24)
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offsets[-1])
+    deg = rng.integers(0, 700, n)
+    indptr = np.concatenate([[0], np.cumsum(deg)])
+    order = block_schedule(indptr, offsets)
+    words = (n + 63) // 64
+    assert order.dtype == np.int32 and sorted(order.tolist()) == list(range(words))
+    # while every group has blocks left, position i takes a block of group (i // 4) % 8, and a group's blocks come
+    # from few instances (each instance lives in one group)
+    binst = np.searchsorted(offsets, order.astype(np.int64) * 64, side="right") - 1
+    owner = {}
+    for i, k in enumerate(binst[: 8 * 4 * 2]):
+        owner.setdefault(int(k), set()).add((i // 4) % 8)
+    assert all(len(g) == 1 for g in owner.values())
 
 
 def test_gpu_entry_points_refuse_the_cpu():
