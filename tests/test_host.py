@@ -160,3 +160,64 @@ def test_gpu_entry_points_refuse_the_cpu():
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+# cap, same-type damping and a fan-in normalisation that actually scales these small graphs
+SHAPING = LIFParams(conn_cap=5.0, same_type_gain=0.5, input_norm_alpha=0.5, input_norm_ref=20.0, path_gain=[],
+                    type_path_gain=[], w_syn=0.275)
+
+
+def test_subset_is_the_induced_subgraph_with_the_original_normalisation():
+    rng = np.random.default_rng(9)
+    c = small_connectome(rng, n=90)
+    p = SHAPING
+    idx = rng.permutation(c.n)[: c.n // 3]
+    sub = c.subset(idx)
+    assert sub.n == len(idx) and sub.reference is c
+    assert (sub.W != c.W[idx][:, idx]).nnz == 0
+    assert np.array_equal(sub.neurons.bodyId.to_numpy(), c.neurons.bodyId.to_numpy()[idx])
+    assert np.array_equal(sub.index_of(sub.neurons.bodyId.to_numpy()[::-1]), np.arange(sub.n)[::-1])
+    # every rule is per edge and the fan-in scale of a row comes from the original graph, so the shaped subset is
+    # the induced subgraph of the shaped original
+    full = shaped_weights(c, p)
+    assert abs(shaped_weights(sub, p) - full[idx][:, idx]).max() == 0
+    # a nested subset keeps the same original
+    nested = sub.subset(np.arange(0, sub.n, 2))
+    assert nested.reference is c
+    assert abs(shaped_weights(nested, p) - full[idx[::2]][:, idx[::2]]).max() == 0
+    # standing alone the subset would normalise by its own (smaller) fan-in
+    alone = Connectome(sub.neurons.copy(), sub.W)
+    assert abs(shaped_weights(alone, p) - full[idx][:, idx]).max() > 0
+    with pytest.raises(ValueError):
+        c.subset([0, 0])
+    with pytest.raises(ValueError):
+        c.subset(np.ones(c.n + 1, bool))
+    assert sub.subset(np.ones(sub.n, bool)).n == sub.n
+
+
+def test_cache_round_trip(tmp_path):
+    rng = np.random.default_rng(10)
+    c = small_connectome(rng, n=60)
+    sub = c.subset(np.arange(5, c.n - 5))
+    sub.save(tmp_path / "cache")
+    back = Connectome.load_cache(tmp_path / "cache")
+    assert (back.W != sub.W).nnz == 0 and back.W.dtype == np.float32
+    assert back.neurons.equals(sub.neurons)
+    assert (back.reference.W != c.W).nnz == 0
+    assert abs(shaped_weights(back, SHAPING) - shaped_weights(sub, SHAPING)).max() == 0
+    c.save(tmp_path / "full")
+    assert not (tmp_path / "full" / "reference_W.npz").exists()
+    assert Connectome.load_cache(tmp_path / "full").reference is not c
+
+
+def test_packed_instances_accept_subsets():
+    rng = np.random.default_rng(11)
+    c = small_connectome(rng, n=90)
+    p = SHAPING
+    parts = [c.subset(np.arange(0, 40)), c.subset(np.arange(40, c.n))]
+    indptr, indices, data = pack_instances(parts, p)
+    ref = sp.block_diag([shaped_weights(x, p) for x in parts], format="csr", dtype=np.float32)
+    ref.sum_duplicates()
+    ref.sort_indices()
+    assert np.array_equal(indptr, ref.indptr) and np.array_equal(indices, ref.indices)
+    assert np.array_equal(data, ref.data)

@@ -334,6 +334,38 @@ def test_fly_frames_bit_exact_with_optic(graphs):
     assert om.st.spike_counts.sum() > 0
 
 
+@pytest.mark.parametrize("graphs", [False, True])
+def test_fly_state_dict_round_trip(graphs):
+    """state_dict (the reference's tensor names, flyverse/fly.py:721-792) mid-run, then load_state_dict into a fresh
+    FlyBrain: both continue with the same bits as the uninterrupted run."""
+    conn, retina, layout, lp, op = fly_setup(0.02, seed=2)
+    idx, hz = synth.fly_inputs(6, layout, 40)
+
+    def make():
+        fb = FlyBrain(conn, retina, batch=2, device=DEV, seed=4, lif_params=lp, optic_params=op, cuda_graphs=graphs)
+        fb.set_poisson(idx, hz)
+        fb.random_movie(23)
+        return fb
+
+    a = make()
+    for _ in range(3):
+        a.step(10.0)
+    state = a.state_dict()
+    assert set(state["brain"]) == set(FlyBrain.BRAIN_TENSORS)
+    for _ in range(2):
+        a.step(10.0)
+    b = make()
+    b.step(10.0)                         # some other history first, then the checkpoint replaces it
+    b.load_state_dict(state)
+    for _ in range(2):
+        b.step(10.0)
+    for k in ("v", "g", "rate", "refrac", "spikes", "spike_counts", "drive"):
+        assert torch.equal(getattr(a.brain, k), getattr(b.brain, k)), k
+    for k in ("v", "adapt", "delta_rate"):
+        assert torch.equal(getattr(a.optic, k), getattr(b.optic, k)), k
+    assert a.brain.spike_counts.sum().item() > 0
+
+
 def _pack_bits(x):
     B, n = x.shape
     nw = (n + 63) // 64
