@@ -7,6 +7,6 @@ and the additions `batch.BatchBrain` (persistent multi-fly stepping) and `adjoin
 from .connectome import Connectome, Retina
 from .brain import Brain, LIFParams
 from .optic import OpticLobe, OpticParams
-from .fly import FlyBrain
+from .fly import FlyBrain, StepResult
 
-__all__ = ["Connectome", "Retina", "Brain", "LIFParams", "OpticLobe", "OpticParams", "FlyBrain"]
+__all__ = ["Connectome", "Retina", "Brain", "LIFParams", "OpticLobe", "OpticParams", "FlyBrain", "StepResult"]

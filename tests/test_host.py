@@ -16,6 +16,7 @@ import scipy.sparse as sp
 from flyhip import hip, synth
 from flyhip.brain import Brain, LIFParams, shaped_weights
 from flyhip.connectome import Connectome
+from flyhip.fly import StepResult, budget_chunk
 from flyhip.packed import PackedBrains, block_schedule, pack_instances
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -221,3 +222,17 @@ def test_packed_instances_accept_subsets():
     ref.sort_indices()
     assert np.array_equal(indptr, ref.indptr) and np.array_equal(indices, ref.indices)
     assert np.array_equal(data, ref.data)
+
+
+def test_budget_chunk_follows_measured_costs():
+    """The wall-budget chunk choice of flyverse/fly.py:640-648: one step before any measurement, then the largest
+    candidate whose measured (or per-step extrapolated) cost fits 1.1x within the remaining time."""
+    cands = [1, 2, 4, 8, 16, 32, 64, 100]
+    assert budget_chunk({}, cands, 0.5) == 1
+    assert budget_chunk({1: 2.0}, cands, 50.0) == 16              # 2 ms/step: 16 * 2.2 = 35.2 <= 50 < 32 * 2.2
+    assert budget_chunk({1: 2.0, 32: 20.0}, cands, 50.0) == 32    # a measured chunk beats the extrapolation
+    assert budget_chunk({4: 4.0}, cands, 4.4) == 4                # smallest measured chunk sets the per-step cost
+    assert budget_chunk({4: 4.0}, cands, 1.0) == 0                # nothing fits
+    assert budget_chunk({1: 0.01}, cands, 1e9) == 100             # the whole frame is the largest candidate
+    r = StepResult(50.0, 25.0, 500)
+    assert r.time_dilation == 2.0 and StepResult(1.0, 0.0, 10).time_dilation == 0.0

@@ -288,6 +288,34 @@ def test_group_means():
     assert np.array_equal(N_(out)[:, 3], np.full(3, 50.0))
 
 
+def test_brain_group_rates_and_spike_total_snapshot():
+    """Brain.group_rates (flyverse/motor.py:232-269): the fixed-order means equal the oracle's, an empty group reads
+    0, and total_spikes reuses the copied spike total until a step or a reset changes the state."""
+    c, inputs, hz = synth.plain_lif_graph(5, n=2000, n_inputs=200)
+    brain = Brain(c, LIFParams.shiu(dt=0.1), device=DEV, batch=3, seed=7)
+    brain.set_poisson(inputs, hz)
+    brain.step(60)
+    rng = np.random.default_rng(4)
+    groups = [rng.integers(0, c.n, 17), np.zeros(0, np.int64), inputs, np.arange(c.n)]
+    with pytest.raises(ValueError):
+        brain.group_rates([np.array([c.n])])
+    out = brain.group_rates(groups)
+    ptr = np.r_[0, np.cumsum([len(g) for g in groups])].astype(np.int32)
+    ref = oracle.group_means(ptr, np.concatenate(groups).astype(np.int32), N_(brain.rate))
+    assert out.shape == (3, 4) and np.array_equal(out, ref) and np.all(out[:, 1] == 0.0)
+    assert out[:, 2].min() > 0
+    spikes = N_(brain.spikes).sum(1)
+    assert np.array_equal(brain.total_spikes(), spikes)
+    brain._spike_total[1][:] = -1.0                       # the snapshot, not a fresh sum, answers until the state moves
+    assert np.all(brain.total_spikes() == -1.0)
+    brain.step()
+    assert np.array_equal(brain.total_spikes(), N_(brain.spikes).sum(1))
+    assert np.array_equal(brain.group_rates(groups[:1]), oracle.group_means(ptr[:2], groups[0].astype(np.int32),
+                                                                            N_(brain.rate)))
+    brain.reset()
+    assert np.all(brain.total_spikes() == 0.0)
+
+
 def test_brain_raster_bit_exact_plain_lif():
     """The plain-LIF workload's statistics at 3,000 neurons: Brain on the GPU against the oracle, 400 steps."""
     c, inputs, hz = synth.plain_lif_graph(4, n=3000, n_inputs=60)
@@ -364,6 +392,44 @@ def test_fly_state_dict_round_trip(graphs):
     for k in ("v", "adapt", "delta_rate"):
         assert torch.equal(getattr(a.optic, k), getattr(b.optic, k)), k
     assert a.brain.spike_counts.sum().item() > 0
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_fly_step_budget(graphs):
+    """step_budget (flyverse/fly.py:618-660): argument checks, the StepResult accounting, and the chunks it took
+    replayed as plain step() calls on a twin brain give the same bits."""
+    conn, retina, layout, lp, op = fly_setup(0.02, seed=3)
+    idx, hz = synth.fly_inputs(6, layout, 40)
+
+    def make():
+        fb = FlyBrain(conn, retina, batch=2, device=DEV, seed=5, lif_params=lp, optic_params=op, cuda_graphs=graphs)
+        fb.set_poisson(idx, hz)
+        fb.random_movie(29)
+        return fb
+
+    a = make()
+    for bad in (dict(wall_ms=-1.0), dict(wall_ms=float("nan")), dict(wall_ms=5.0, frame_ms=0.05)):
+        with pytest.raises(ValueError):
+            a.step_budget(**bad)
+    chunks = []
+    total = 0
+    for wall in (40.0, 0.0, 60.0):
+        before = a.brain.step_count
+        r = a.step_budget(wall, frame_ms=2.0)
+        chunks += a._budget_chunks
+        assert r.steps == a.brain.step_count - before == sum(a._budget_chunks)
+        assert r.simulated_ms == pytest.approx(r.steps * lp.dt, abs=1e-9)
+        assert r.wall_ms >= 0 and (r.steps > 0) == (wall > 0)
+        assert all(1 <= n <= 20 for n in a._budget_chunks)
+        total += r.steps
+    b = make()
+    for n in chunks:
+        b.step(n * lp.dt)
+    assert b.brain.step_count == a.brain.step_count == total
+    for k in ("v", "g", "rate", "refrac", "spikes", "spike_counts", "drive"):
+        assert torch.equal(getattr(a.brain, k), getattr(b.brain, k)), k
+    for k in ("v", "adapt", "delta_rate"):
+        assert torch.equal(getattr(a.optic, k), getattr(b.optic, k)), k
 
 
 def _pack_bits(x):
