@@ -684,27 +684,45 @@ __global__ __launch_bounds__(256) void k_optic_dr_fm(const float* __restrict__ v
     dr[t] = fh_clip01(s) - base;
 }
 
-// Photoreceptor stage with the radiance drawn in-kernel: rad[p, c] of lane b = Philox(movie_seed[b], frame[b],
-// p*4 + c, 0, stream 1), the same uniform the plain path's launch_uniform gives fly 0 of a one-fly batch.
-__global__ __launch_bounds__(256) void k_photoreceptor_fm(const float* __restrict__ sens, float* __restrict__ I_lp,
+// Photoreceptor stage over (column, family) cells with the radiance drawn in-kernel, one thread per (cell k, lane b):
+// rad[col, c] of lane b = Philox(movie_seed[b], frame[b], col*4 + c, 0, stream 1), the uniform the plain path's
+// launch_uniform gives fly 0 of a one-fly batch; the averaging, state update and contrast are those of
+// csrc/optic.hip's k_photoreceptor.  I_lp, I_mean (5 n_col, 64); a_pr (n_pr, 64).
+__global__ __launch_bounds__(256) void k_photoreceptor_fm(const float* __restrict__ sens, const int* __restrict__ pr_col,
+                                                          const int* __restrict__ cell_ptr,
+                                                          const int* __restrict__ cell_pr,
+                                                          const float* __restrict__ cell_w, float* __restrict__ I_lp,
                                                           float* __restrict__ I_mean, float* __restrict__ a_pr,
                                                           const uint64_t* __restrict__ movie_seed,
                                                           const uint32_t* __restrict__ frame,
                                                           const uint64_t* __restrict__ fresh_mask,
                                                           const uint64_t* __restrict__ live_mask,
-                                                          const float* __restrict__ cf, int n_pr) {
+                                                          const float* __restrict__ cf, int n_cells) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n_pr * 64) return;
-    const int lane = (int)(t & 63), p = (int)(t >> 6);
+    if (t >= (int64_t)n_cells * 64) return;
+    const int lane = (int)(t & 63), k = (int)(t >> 6);
     if (!((*live_mask >> lane) & 1ull)) return;
     const uint64_t ms = movie_seed[lane];
     const uint32_t fr = frame[lane];
+    const int e0 = cell_ptr[k], e1 = cell_ptr[k + 1];
     float I = 0.0f;
+    for (int cs = e0; cs < e1; cs += 16) {
+        const int ce = min(cs + 16, e1);
+        float s = 0.0f;
+        for (int e = cs; e < ce; ++e) {
+            const int p = cell_pr[e];
+            const uint32_t base = (uint32_t)pr_col[p] * 4u;
+            float Ip = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        float rad = fh_uniform(ms, fr, (uint32_t)(p * 4 + c), 0u, 1u);
-        float q = rad * sens[p * 4 + c];
-        I = I + q;
+            for (int c = 0; c < 4; ++c) {
+                float rad = fh_uniform(ms, fr, base + (uint32_t)c, 0u, 1u);
+                float q = rad * sens[p * 4 + c];
+                Ip = Ip + q;
+            }
+            float q = cell_w[e] * Ip;
+            s = s + q;
+        }
+        I = I + s;
     }
     const bool fr0 = (*fresh_mask >> lane) & 1ull;
     float lp = fr0 ? I : I_lp[t];
@@ -718,7 +736,7 @@ __global__ __launch_bounds__(256) void k_photoreceptor_fm(const float* __restric
     float c = num / den;
     c = c < -1.0f ? -1.0f : c;
     c = c > cf[5] ? cf[5] : c;
-    a_pr[t] = c;
+    for (int e = e0; e < e1; ++e) a_pr[(int64_t)cell_pr[e] * 64 + lane] = c;
 }
 
 __global__ __launch_bounds__(256) void k_feedback_fm(const float* __restrict__ rate, const int* __restrict__ spk_row,
@@ -858,13 +876,15 @@ FH_EXPORT int launch_optic_dr_fm(void** p, int rows, void* stream) {
     FH_LAUNCH_CHECK();
 }
 
-// p = [sens, I_lp, I_mean, a_pr, movie_seed, frame, fresh_mask, live_mask, coefs].
-FH_EXPORT int launch_photoreceptor_fm(void** p, int n_pr, void* stream) {
-    if (n_pr < 0) return FH_ERR_ARGS;
-    if (!n_pr) return 0;
-    hipLaunchKernelGGL(k_photoreceptor_fm, dim3(nblk((int64_t)n_pr * 64)), dim3(256), 0, (hipStream_t)stream,
-                       (const float*)p[0], (float*)p[1], (float*)p[2], (float*)p[3], (const uint64_t*)p[4],
-                       (const uint32_t*)p[5], (const uint64_t*)p[6], (const uint64_t*)p[7], (const float*)p[8], n_pr);
+// p = [sens (n_pr, 4), pr_col, cell_ptr (n_cells + 1), cell_pr, cell_w, I_lp, I_mean, a_pr, movie_seed, frame,
+// fresh_mask, live_mask, coefs]; n_cells = 5 * columns.
+FH_EXPORT int launch_photoreceptor_fm(void** p, int n_cells, void* stream) {
+    if (n_cells < 0) return FH_ERR_ARGS;
+    if (!n_cells) return 0;
+    hipLaunchKernelGGL(k_photoreceptor_fm, dim3(nblk((int64_t)n_cells * 64)), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)p[0], (const int*)p[1], (const int*)p[2], (const int*)p[3], (const float*)p[4],
+                       (float*)p[5], (float*)p[6], (float*)p[7], (const uint64_t*)p[8], (const uint32_t*)p[9],
+                       (const uint64_t*)p[10], (const uint64_t*)p[11], (const float*)p[12], n_cells);
     FH_LAUNCH_CHECK();
 }
 

@@ -38,19 +38,48 @@ __global__ __launch_bounds__(256) void k_optic_update(OpticArgs s, int n, int64_
     s.dr[t] = fh_clip01(q) - s.b[i];
 }
 
-// Photoreceptor stage with one (column, family) cell per photoreceptor.  rad (B, n_pr, 4), sens (n_pr, 4),
-// I_lp / I_mean (B, n_pr), fresh (B) uint8 flags, coefs = [a_lp, 1-a_lp, a_ad, 1-a_ad, eps, contrast_clip].
-// fresh is cleared by the caller after the launch (launch_photoreceptor does it in a second tiny kernel).
-__global__ __launch_bounds__(256) void k_photoreceptor(const float* __restrict__ rad, const float* __restrict__ sens,
-                                                       float* __restrict__ I_lp, float* __restrict__ I_mean,
-                                                       const uint8_t* __restrict__ fresh, const float* __restrict__ cf,
-                                                       float* __restrict__ a_pr, int n_pr, int64_t total) {
+// Photoreceptor stage over (column, family) cells (flyverse/optic.py:496-517), one thread per (fly b, cell k),
+// cell = column * 5 + family.  The cell's input is its row of the averaging matrix (optic.py:320-325: weights
+// cell_w = 1/count, photoreceptors ascending) applied to the intensities I_p = sum_c rad[b, column(p), c] * sens[p, c]
+// (channel order, from +0), in the CSR products' order: chunks of 16 folded from +0, chunk sums folded from +0.
+// Then the low-pass and adaptation state of the cell, its contrast (0 for a cell without photoreceptors,
+// optic.py:513-514), and that contrast as the activity of each of its photoreceptors (optic.py:517).
+// rad (B, n_col, 4); I_lp, I_mean, contrast (B, n_cells); a_pr (B, n_pr); fresh (B) uint8 flags, cleared by the
+// launch's second kernel; coefs = [a_lp, 1-a_lp, a_ad, 1-a_ad, eps, contrast_clip].
+struct PrCells {
+    const float* rad;
+    const float* sens;
+    const int* pr_col;
+    const int* cell_ptr;
+    const int* cell_pr;
+    const float* cell_w;
+};
+
+__global__ __launch_bounds__(256) void k_photoreceptor(PrCells pc, float* __restrict__ I_lp,
+                                                       float* __restrict__ I_mean, const uint8_t* __restrict__ fresh,
+                                                       const float* __restrict__ cf, float* __restrict__ contrast,
+                                                       float* __restrict__ a_pr, int n_col, int n_pr,
+                                                       int64_t total) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    int p = (int)(t % n_pr), b = (int)(t / n_pr);
+    const int n_cells = n_col * 5;
+    const int k = (int)(t % n_cells), b = (int)(t / n_cells);
+    const int e0 = pc.cell_ptr[k], e1 = pc.cell_ptr[k + 1];
     float I = 0.0f;
+    for (int cs = e0; cs < e1; cs += 16) {
+        const int ce = min(cs + 16, e1);
+        float s = 0.0f;
+        for (int e = cs; e < ce; ++e) {
+            const int p = pc.cell_pr[e];
+            const float* r = pc.rad + ((int64_t)b * n_col + pc.pr_col[p]) * 4;
+            float Ip = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) { float q = rad[t * 4 + c] * sens[p * 4 + c]; I = I + q; }
+            for (int c = 0; c < 4; ++c) { float q = r[c] * pc.sens[p * 4 + c]; Ip = Ip + q; }
+            float q = pc.cell_w[e] * Ip;
+            s = s + q;
+        }
+        I = I + s;
+    }
     bool fr = fresh[b] != 0;
     float lp = fr ? I : I_lp[t];
     float mean = fr ? I : I_mean[t];
@@ -63,7 +92,9 @@ __global__ __launch_bounds__(256) void k_photoreceptor(const float* __restrict__
     float c = num / den;
     c = c < -1.0f ? -1.0f : c;
     c = c > cf[5] ? cf[5] : c;
-    a_pr[t] = c;
+    if (e1 == e0) c = 0.0f;
+    contrast[t] = c;
+    for (int e = e0; e < e1; ++e) a_pr[(int64_t)b * n_pr + pc.cell_pr[e]] = c;
 }
 
 __global__ void k_clear_u8(uint8_t* f, int n) {
@@ -133,15 +164,19 @@ FH_EXPORT int launch_optic(void** p, int n, int batch, float gr, float ga, float
     FH_LAUNCH_CHECK();
 }
 
-// Photoreceptor stage: p = [rad (B, n_pr, 4), sens (n_pr, 4), I_lp, I_mean, fresh uint8 (B), coefs (6), a_pr].
-FH_EXPORT int launch_photoreceptor(void** p, int n_pr, int batch, void* stream) {
-    if (n_pr < 0 || batch < 0) return FH_ERR_ARGS;
-    int64_t total = (int64_t)n_pr * batch;
+// Photoreceptor stage: p = [rad (B, n_col, 4), sens (n_pr, 4), pr_col int32 (n_pr), cell_ptr int32 (5 n_col + 1),
+// cell_pr int32 (n_pr), cell_w (n_pr), I_lp (B, 5 n_col), I_mean (B, 5 n_col), fresh uint8 (B), coefs (6),
+// contrast (B, 5 n_col), a_pr (B, n_pr)].
+FH_EXPORT int launch_photoreceptor(void** p, int n_col, int n_pr, int batch, void* stream) {
+    if (n_col < 0 || n_pr < 0 || batch < 0) return FH_ERR_ARGS;
+    int64_t total = (int64_t)n_col * 5 * batch;
     if (!total) return 0;
-    hipLaunchKernelGGL(k_photoreceptor, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
-                       (const float*)p[1], (float*)p[2], (float*)p[3], (const uint8_t*)p[4], (const float*)p[5],
-                       (float*)p[6], n_pr, total);
-    hipLaunchKernelGGL(k_clear_u8, dim3((batch + 255) / 256), dim3(256), 0, (hipStream_t)stream, (uint8_t*)p[4], batch);
+    PrCells pc{(const float*)p[0], (const float*)p[1], (const int*)p[2], (const int*)p[3], (const int*)p[4],
+               (const float*)p[5]};
+    hipLaunchKernelGGL(k_photoreceptor, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, pc, (float*)p[6],
+                       (float*)p[7], (const uint8_t*)p[8], (const float*)p[9], (float*)p[10], (float*)p[11], n_col,
+                       n_pr, total);
+    hipLaunchKernelGGL(k_clear_u8, dim3((batch + 255) / 256), dim3(256), 0, (hipStream_t)stream, (uint8_t*)p[8], batch);
     FH_LAUNCH_CHECK();
 }
 

@@ -89,9 +89,12 @@ int launch_spmv_grouped(void** p, int rows, int cols, int ngroups, int batch, fl
 int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batch, int range_cols, float scale, int accumulate,
                     void* stream);
 
-/* Photoreceptor low-pass, adaptation and contrast, one cell per photoreceptor (flyverse/optic.py:496-517).
- * p = [rad (batch, n_pr, 4), sens (n_pr, 4), I_lp, I_mean, fresh uint8 (batch), coefs(6), a_pr]; clears fresh. */
-int launch_photoreceptor(void** p, int n_pr, int batch, void* stream);
+/* Photoreceptor stage per (column, family) cell (flyverse/optic.py:496-517): the averaging matrix of
+ * optic.py:320-325 (cell_w = 1/count) over the photoreceptor intensities, low-pass, adaptation and contrast (0 for an
+ * empty cell), each photoreceptor's activity = its cell's contrast.  p = [rad (batch, n_col, 4), sens (n_pr, 4),
+ * pr_col int32 (n_pr), cell_ptr int32 (5 n_col + 1), cell_pr int32 (n_pr), cell_w (n_pr), I_lp, I_mean
+ * (batch, 5 n_col), fresh uint8 (batch), coefs (6), contrast (batch, 5 n_col), a_pr (batch, n_pr)]; clears fresh. */
+int launch_photoreceptor(void** p, int n_col, int n_pr, int batch, void* stream);
 
 /* s = clamp(rate[:, spk_idx] / 100, 0, 3), flyverse/optic.py:572.  p = [rate (batch, n), spk_idx, s]. */
 int launch_feedback(void** p, int n, int n_spk, int batch, void* stream);
@@ -132,9 +135,10 @@ int launch_optic_substep_fm(void** p, int rows, float gr, float ga, float aad, i
 /* dr = clip(v + b, 0, 1) - b on (rows, 64).  p = [v, b, dr].  launch_optic_dr in the fly-minor layout. */
 int launch_optic_dr_fm(void** p, int rows, void* stream);
 
-/* Photoreceptor stage with the movie drawn in-kernel (Philox, stream 1).  p = [sens, I_lp, I_mean, a_pr,
- * movie_seed, frame, fresh_mask, live_mask, coefs].  launch_photoreceptor in the fly-minor layout. */
-int launch_photoreceptor_fm(void** p, int n_pr, void* stream);
+/* Photoreceptor stage with the movie drawn in-kernel (Philox, stream 1, counter column * 4 + channel).  p = [sens,
+ * pr_col, cell_ptr, cell_pr, cell_w, I_lp (5 n_col, 64), I_mean, a_pr (n_pr, 64), movie_seed, frame, fresh_mask,
+ * live_mask, coefs]; n_cells = 5 n_col.  launch_photoreceptor in the fly-minor layout. */
+int launch_photoreceptor_fm(void** p, int n_cells, void* stream);
 
 /* s = clamp(rate / 100, 0, 3) for the spiking rows.  p = [rate (rows, 64), spk_row (n_spk), s (n_spk, 64)]. */
 int launch_feedback_fm(void** p, int n_spk, void* stream);

@@ -21,7 +21,9 @@ Pinning (what checks each function against something other than itself), by func
   spmv           scipy float64 M @ x within 1e-5 relative; the chunk order against a pure-Python fold
                  (tests/test_oracle.py); half rounding against numpy float16 casting.
   optic_dr/update  the torch float32 expressions of flyverse/optic.py:528-551 evaluated op by op (exact).
-  photoreceptor  the torch float32 expressions of flyverse/optic.py:505-513 evaluated op by op (exact).
+  photoreceptor  the torch float32 expressions of flyverse/optic.py:503-517 evaluated op by op on the reference's
+                 averaging matrix (optic.py:320-325), its product folded in the CSR order (exact) and torch's own
+                 sparse product within 1e-6; float64 numpy with that matrix within 1e-5 relative (tests/test_oracle.py).
   group_means    numpy float64 means.
 """
 from __future__ import annotations
@@ -66,7 +68,8 @@ def lib():
         L.orc_spmv.argtypes = [_I, _I, _I, _i64p, _i32p, _f32p, _f32p, _f32p, _I]
         L.orc_optic_dr.argtypes = [_I, _I, _f32p, _f32p, _f32p]
         L.orc_optic_update.argtypes = [_I, _I] + [_f32p] * 8 + [_F, _F, _F]
-        L.orc_photoreceptor.argtypes = [_I, _I, _f32p, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p]
+        L.orc_photoreceptor.argtypes = [_I, _I, _I, _f32p, _f32p, _i32p, _i32p, _i32p, _f32p, _f32p, _f32p, _u8p,
+                                        _f32p, _f32p, _f32p]
         L.orc_group_means.argtypes = [_I, _I, _I, _I, _i32p, _i32p, _f32p, _f64p]
         L.orc_drive_out.argtypes = [_I, _I, _I, _f32p, _i32p, _F, _F, _f32p]
         L.orc_feedback.argtypes = [_I, _I, _I, _f32p, _i32p, _f32p]
@@ -138,10 +141,27 @@ def optic_update(v, adapt, dr, y, pr, spk, a, b, gain_rr, adapt_gain, a_ad):
                            np.float32(adapt_gain), np.float32(a_ad))
 
 
-def photoreceptor(rad, sens, I_lp, I_mean, fresh, coefs):
-    B, n_pr = I_lp.shape
-    a_pr = np.empty_like(I_lp)
-    lib().orc_photoreceptor(n_pr, B, _f32(rad), _f32(sens), I_lp, I_mean, fresh, _f32(coefs), a_pr)
+def identity_cells(n_pr):
+    """The cell tables of a retina with one photoreceptor per column, all of family 0 (cell = column * 5)."""
+    return {"cell_ptr": np.repeat(np.arange(n_pr + 1, dtype=np.int32), [1] + [5] * n_pr)[:5 * n_pr + 1],
+            "cell_pr": np.arange(n_pr, dtype=np.int32), "cell_w": np.ones(n_pr, np.float32),
+            "pr_column": np.arange(n_pr, dtype=np.int32)}
+
+
+def photoreceptor(rad, sens, I_lp, I_mean, fresh, coefs, cells=None, contrast=None):
+    """rad (B, n_col, 4); I_lp, I_mean (B, n_col*5) updated in place; returns a_pr (B, n_pr).  `cells` holds the
+    averaging tables (cell_ptr, cell_pr, cell_w, pr_column), by default those of identity_cells."""
+    B, n_col = rad.shape[:2]
+    n_pr = sens.shape[0]
+    cells = identity_cells(n_pr) if cells is None else cells
+    if I_lp.shape != (B, 5 * n_col) or len(cells["cell_ptr"]) != 5 * n_col + 1:
+        raise ValueError("photoreceptor state must be (batch, 5 * columns)")
+    a_pr = np.zeros((B, n_pr), np.float32)
+    contrast = np.empty((B, 5 * n_col), np.float32) if contrast is None else contrast
+    lib().orc_photoreceptor(n_col, n_pr, B, _f32(rad), _f32(sens), np.ascontiguousarray(cells["pr_column"], np.int32),
+                            np.ascontiguousarray(cells["cell_ptr"], np.int32),
+                            np.ascontiguousarray(cells["cell_pr"], np.int32), _f32(cells["cell_w"]), I_lp, I_mean,
+                            fresh, _f32(coefs), contrast, a_pr)
     return a_pr
 
 
@@ -219,12 +239,12 @@ def brain_step(st: BrainState, csc, P, a_s, seed, flags, poisson_on=True, record
 
 
 class OpticState:
-    def __init__(self, B, n_rate, n_pr):
+    def __init__(self, B, n_rate, n_cells):
         self.v = np.zeros((B, n_rate), np.float32)
         self.adapt = np.zeros((B, n_rate), np.float32)
         self.dr = np.zeros((B, n_rate), np.float32)
-        self.I_lp = np.zeros((B, n_pr), np.float32)
-        self.I_mean = np.zeros((B, n_pr), np.float32)
+        self.I_lp = np.zeros((B, n_cells), np.float32)
+        self.I_mean = np.zeros((B, n_cells), np.float32)
         self.fresh = np.ones(B, np.uint8)
         self.pending_ms = 0.0
 
@@ -232,7 +252,7 @@ class OpticState:
 def optic_frame(os_: OpticState, g, rad, rate, frame_ms):
     """OpticLobe._step_frame_inference (flyverse/optic.py:563-604) on graph `g` (a flyhip.synth.FlyGraph): returns the
     (B, N) drive written into the LIF (fly.py:492)."""
-    a_pr = photoreceptor(rad, g.pr_sens, os_.I_lp, os_.I_mean, os_.fresh, g.pr_coefs(frame_ms))
+    a_pr = photoreceptor(rad, g.pr_sens, os_.I_lp, os_.I_mean, os_.fresh, g.pr_coefs(frame_ms), g.cells)
     s = feedback(rate, g.spk_idx)
     total = os_.pending_ms + frame_ms
     steps = int(np.floor((total + 1e-9) / g.optic.dt_ms))
@@ -257,12 +277,15 @@ def optic_frame(os_: OpticState, g, rad, rate, frame_ms):
 # ------------------------------------------------------------------ the whole fly (flyverse/fly.py:448-507)
 class OptSpec:
     """Host-built optic-lobe inputs of one graph (the arrays flyhip.optic.optic_matrices produces)."""
-    def __init__(self, m: dict, params, sens, n):
+    def __init__(self, m: dict, params, sens, n, cells=None):
         to = lambda M: (M.indptr.astype(np.int64), M.indices.astype(np.int32), M.data.astype(np.float32))
         self.W_rr, self.W_rp, self.W_rs, self.W_sr = (to(m[k]) for k in ("M_rr", "M_rp", "M_rs", "M_sr"))
         self.a_vec, self.b_vec, self.a_ad = m["a"], m["b"], m["a_ad"]
         self.spk_idx = m["spk_idx"].astype(np.int32)
         self.pr_sens = np.ascontiguousarray(sens, np.float32)
+        self.cells = identity_cells(len(self.pr_sens)) if cells is None else cells
+        self.n_cells = len(self.cells["cell_ptr"]) - 1
+        self.n_col = self.n_cells // 5
         self.optic = params
         self.n = n
 
@@ -287,7 +310,7 @@ class FlyModel:
         self.st.active = np.ascontiguousarray(active, np.float32)
         self.P, self.a_s, self.csc, self.seeds, self.dt = _f32(P), float(a_s), csc, seeds, dt
         self.optic = optic
-        self.os = OpticState(B, len(optic.b_vec), len(optic.pr_sens)) if optic is not None else None
+        self.os = OpticState(B, len(optic.b_vec), optic.n_cells) if optic is not None else None
         self.movie_seeds = movie_seeds
         self.frame_index = 0
         self.B, self.n = B, n
@@ -314,8 +337,8 @@ class FlyModel:
     def frame(self, steps, raster=None):
         """One frame: optic first (if any) with a fresh random movie, then `steps` LIF steps; appends spikes."""
         if self.optic is not None:
-            npr = len(self.optic.pr_sens)
-            rad = _per_fly_uniform(self.B, npr * 4, self.movie_seeds, self.frame_index, 1).reshape(self.B, npr, 4)
+            ncol = self.optic.n_col
+            rad = _per_fly_uniform(self.B, ncol * 4, self.movie_seeds, self.frame_index, 1).reshape(self.B, ncol, 4)
             self.st.drive[:] = optic_frame(self.os, self.optic, rad, self.st.rate, steps * self.dt)
         for st in self.lif_steps(steps):
             if raster is not None:

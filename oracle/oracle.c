@@ -230,19 +230,37 @@ void orc_optic_update(int n, int batch, float* v, float* adapt, float* dr, const
 }
 
 /* ------------------------------------------------------------------ photoreceptor stage (A10)
- * flyverse/optic.py:496-517 with the (column, family) averaging matrix taken as the identity (one cell per
- * photoreceptor, which is how the synthetic graphs place them).  Intensity = sum over the 4 spectral channels in
- * channel order (sequential fold from +0); first frame of a row initialises the low-pass and the mean to I
- * (optic.py:505-508); then I_lp = a_lp*I_lp + (1-a_lp)*I; I_mean = a_ad*I_mean + (1-a_ad)*I_lp;
- * contrast = clamp((I_lp - I_mean)/(I_mean + eps), -1, clip).  coefs = [a_lp, 1-a_lp, a_ad, 1-a_ad, eps, clip]
- * as float32 (the host computes them in double, optic.py:509). */
-void orc_photoreceptor(int n_pr, int batch, const float* rad, const float* sens, float* I_lp, float* I_mean,
-                       uint8_t* fresh, const float* coefs, float* a_pr) {
+ * flyverse/optic.py:496-517 over (column, family) cells, cell = column * 5 + family (optic.py:315-325).  For batch
+ * row b and cell k: each photoreceptor p of the cell has intensity I_p = sum over the 4 spectral channels, in
+ * channel order, of rad[b, column(p), c] * sens[p, c] (sequential fold from +0); the cell's input is the averaging
+ * matrix row applied to I (weights cell_w = 1/count as float32, photoreceptors ascending) in the CSR product's
+ * chunked order (chunks of 16 folded from +0, chunk sums folded from +0); the first frame of a row initialises the
+ * low-pass and the mean to that input (optic.py:505-508); then I_lp = a_lp*I_lp + (1-a_lp)*I;
+ * I_mean = a_ad*I_mean + (1-a_ad)*I_lp; contrast = clamp((I_lp - I_mean)/(I_mean + eps), -1, clip), replaced by 0
+ * for a cell without photoreceptors (optic.py:513-514); each photoreceptor's activity is its cell's contrast
+ * (optic.py:517).  coefs = [a_lp, 1-a_lp, a_ad, 1-a_ad, eps, clip] as float32 (the host computes them in double,
+ * optic.py:509).  rad (B, n_col, 4); I_lp, I_mean, contrast (B, n_col*5); a_pr (B, n_pr). */
+void orc_photoreceptor(int n_col, int n_pr, int batch, const float* rad, const float* sens, const int32_t* pr_col,
+                       const int32_t* cell_ptr, const int32_t* cell_pr, const float* cell_w, float* I_lp,
+                       float* I_mean, uint8_t* fresh, const float* coefs, float* contrast, float* a_pr) {
+    const int n_cells = n_col * 5;
     for (int b = 0; b < batch; ++b) {
-        for (int p = 0; p < n_pr; ++p) {
-            int64_t t = (int64_t)b * n_pr + p;
+        for (int k = 0; k < n_cells; ++k) {
+            int64_t t = (int64_t)b * n_cells + k;
             float I = 0.0f;
-            for (int c = 0; c < 4; ++c) { float q = rad[t * 4 + c] * sens[p * 4 + c]; I = I + q; }
+            for (int cs = cell_ptr[k]; cs < cell_ptr[k + 1]; cs += ORC_CHUNK) {
+                int ce = cs + ORC_CHUNK < cell_ptr[k + 1] ? cs + ORC_CHUNK : cell_ptr[k + 1];
+                float s = 0.0f;
+                for (int e = cs; e < ce; ++e) {
+                    int p = cell_pr[e];
+                    const float* r = rad + ((int64_t)b * n_col + pr_col[p]) * 4;
+                    float Ip = 0.0f;
+                    for (int c = 0; c < 4; ++c) { float q = r[c] * sens[p * 4 + c]; Ip = Ip + q; }
+                    float q = cell_w[e] * Ip;
+                    s = s + q;
+                }
+                I = I + s;
+            }
             float lp = fresh[b] ? I : I_lp[t];
             float mean = fresh[b] ? I : I_mean[t];
             float l1 = coefs[0] * lp, l2 = coefs[1] * I;
@@ -254,7 +272,9 @@ void orc_photoreceptor(int n_pr, int batch, const float* rad, const float* sens,
             float c = num / den;
             if (c < -1.0f) c = -1.0f;
             if (c > coefs[5]) c = coefs[5];
-            a_pr[t] = c;
+            if (cell_ptr[k + 1] == cell_ptr[k]) c = 0.0f;
+            contrast[t] = c;
+            for (int e = cell_ptr[k]; e < cell_ptr[k + 1]; ++e) a_pr[(int64_t)b * n_pr + cell_pr[e]] = c;
         }
     }
     for (int b = 0; b < batch; ++b) fresh[b] = 0;

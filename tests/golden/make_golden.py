@@ -70,7 +70,8 @@ def c1_oracle():
     return {"config": C1, "spike_totals": totals, "raster_sha256": raster_digest(rows),
             "frame_totals": [sum(totals[k * fs:(k + 1) * fs]) for k in range(C1["frames"])],
             "state_sha256": digest(s.v, s.g, s.refrac, s.rate, s.spike_counts),
-            "optic_sha256": digest(o.v, o.adapt, o.I_lp, o.I_mean)}
+            # the photoreceptor state as seen by each photoreceptor (its (column, family) cell's values)
+            "optic_sha256": digest(o.v, o.adapt, *(x[:, retina.cells()["pr_cell"]] for x in (o.I_lp, o.I_mean)))}
 
 
 if __name__ == "__main__":

@@ -7,6 +7,7 @@ import scipy.sparse as sp
 import oracle
 from flyhip import synth
 from flyhip.brain import LIFParams, shaped_weights
+from flyhip.connectome import Retina
 from flyhip.optic import OpticParams, optic_matrices
 
 
@@ -50,6 +51,15 @@ def oracle_fly(conn, retina, lp, op, B, seeds, movie_seeds, with_optic=True):
     csc = (Wc.indptr.astype(np.int64), Wc.indices.astype(np.int32), Wc.data.astype(np.float32))
     P = oracle.lif_params_vector(lp)
     n_delay = max(1, int(round(lp.delay / lp.dt)))
-    spec = oracle.OptSpec(m, op, retina.pr_sens, conn.n) if m is not None else None
+    spec = oracle.OptSpec(m, op, retina.pr_sens, conn.n, retina.cells()) if m is not None else None
     return oracle.FlyModel(conn.n, B, P, math.exp(-lp.dt / lp.tau_syn), n_delay, csc, active, seeds, spec,
                            movie_seeds, v_rest=lp.v_rest, dt=lp.dt)
+
+
+def grouped_retina(retina, per_column=3, seed=0):
+    """The same photoreceptors placed `per_column` to a column in a shuffled order, with random spectral families and
+    two columns left empty: cells that average several photoreceptors (flyverse/optic.py:320-325)."""
+    rng = np.random.default_rng(seed)
+    n = len(retina.pr_index)
+    col = rng.permutation(n) // per_column
+    return Retina(retina.pr_index, retina.pr_sens, col, rng.choice(5, n, p=[.6, .1, .1, .1, .1]), int(col.max()) + 3)

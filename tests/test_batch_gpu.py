@@ -7,7 +7,7 @@ import torch
 
 from flyhip import synth
 from flyhip.batch import BatchBrain, Rollout
-from helpers import fly_setup, oracle_fly
+from helpers import fly_setup, grouped_retina, oracle_fly
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
@@ -29,10 +29,14 @@ def rollouts(layout, n, frames_lo, frames_hi, frame_steps, seed=0):
     return out
 
 
-@pytest.mark.parametrize("graph,mode", [(False, 0), (True, 1), (True, 2), (False, 2), (True, 3), (False, 3), (True, 4), (False, 4),
-                                        (True, 5), (False, 5)])
-def test_lanes_equal_solo_oracle_runs_with_refill(graph, mode):
+@pytest.mark.parametrize("graph,mode,grouped", [(False, 0, False), (True, 1, False), (True, 2, False), (False, 2, False),
+                                                (True, 3, False), (False, 3, False), (True, 4, False),
+                                                (False, 4, False), (True, 5, False), (False, 5, False),
+                                                (True, 5, True)])
+def test_lanes_equal_solo_oracle_runs_with_refill(graph, mode, grouped):
     conn, retina, layout, lp, op = fly_setup(0.02, seed=2)
+    if grouped:
+        retina = grouped_retina(retina)
     frame_ms, frame_steps = 1.0, 10           # 10 LIF steps and one optic substep per frame: the ring holds a frame
     bb = BatchBrain(conn, retina, lp, op, device=DEV, frame_ms=frame_ms, grid=64, fold_mode=mode)
     rs = rollouts(layout, 70, 3, 5, frame_steps, seed=1)    # 70 rollouts > 64 lanes: six lanes are refilled

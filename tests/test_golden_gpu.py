@@ -66,5 +66,6 @@ def test_full_scale_fly_frames_match_golden():
     b, o = fb.brain, fb.optic
     st = [getattr(b, k)[0].cpu().numpy() for k in ("v", "g", "refrac", "rate", "spike_counts")]
     assert mg.digest(*st) == want["state_sha256"]
-    ost = [t[0].cpu().numpy() for t in (o.v, o.adapt, o.I_lp, o.I_mean)]
+    pr_cell = torch.as_tensor(retina.cells()["pr_cell"].astype(np.int64), device=o.I_lp.device)
+    ost = [t[0].cpu().numpy() for t in (o.v, o.adapt, o.I_lp[:, pr_cell], o.I_mean[:, pr_cell])]
     assert mg.digest(*ost) == want["optic_sha256"]

@@ -10,6 +10,7 @@ import scipy.sparse as sp
 import torch
 
 import oracle
+from flyhip.connectome import Retina
 from helpers import random_csr
 
 
@@ -251,7 +252,7 @@ def test_photoreceptor_matches_reference_torch_ops():
     rng = np.random.default_rng(5)
     B, n = 2, 300
     sens = rng.random((n, 4)).astype(np.float32)
-    I_lp, I_mean = np.zeros((B, n), np.float32), np.zeros((B, n), np.float32)
+    I_lp, I_mean = np.zeros((B, 5 * n), np.float32), np.zeros((B, 5 * n), np.float32)
     fresh = np.ones(B, np.uint8)
     tl, tm = torch.zeros(B, n), torch.zeros(B, n)
     tfresh = torch.ones(B, dtype=torch.bool)
@@ -274,6 +275,82 @@ def test_photoreceptor_matches_reference_torch_ops():
         contrast = ((tl - tm) / (tm + 0.02)).clamp(-1.0, 2.0)
         assert torch.equal(torch.from_numpy(got), contrast)
     assert fresh.sum() == 0
+
+
+def _reference_photoreceptor_torch(retina, rad, tl, tm, tfresh, coefs):
+    """flyverse/optic.py:496-517 written with torch float32 ops on the reference's own averaging matrix
+    (optic.py:315-325); the channel sum in channel order."""
+    pr_col, fam = retina.pr_column, retina.pr_family
+    n_col, n_pr = retina.n_col, len(pr_col)
+    cnt = np.zeros((n_col, 5), np.float32)
+    np.add.at(cnt, (pr_col, fam), 1.0)
+    rows = pr_col * 5 + fam
+    A = sp.csr_matrix((1.0 / cnt[pr_col, fam], (rows, np.arange(n_pr))), shape=(n_col * 5, n_pr))
+    At = torch.sparse_csr_tensor(torch.from_numpy(A.indptr.astype(np.int64)), torch.from_numpy(A.indices.astype(np.int64)),
+                                 torch.from_numpy(A.data.astype(np.float32)), size=A.shape)
+    has = np.zeros((n_col, 5), bool)
+    np.add.at(has, (pr_col, fam), True)
+    r = torch.from_numpy(rad)[:, torch.from_numpy(pr_col)]
+    sens = torch.from_numpy(retina.pr_sens)
+    I_pr = r[:, :, 0] * sens[None, :, 0]
+    for c in range(1, 4):
+        I_pr = I_pr + r[:, :, c] * sens[None, :, c]
+    # the averaging product in the order every CSR product of this build defines (its rows here are shorter than
+    # one 16-entry chunk: a sequential fold from +0, each product rounded); torch's own sparse product on the CPU
+    # (MKL, fused multiply-adds) is the reference's arithmetic and agrees to rounding
+    I = torch.zeros(I_pr.shape[0], A.shape[0])
+    for k in range(A.shape[0]):
+        for e in range(A.indptr[k], A.indptr[k + 1]):
+            I[:, k] = I[:, k] + torch.tensor(A.data[e], dtype=torch.float32) * I_pr[:, A.indices[e]]
+    torch.testing.assert_close((At @ I_pr.T.contiguous()).T, I, rtol=1e-6, atol=0)
+    torch.where(tfresh[:, None], I, tl, out=tl)
+    torch.where(tfresh[:, None], I, tm, out=tm)
+    tfresh[:] = False
+    k = [float(x) for x in coefs]                   # python floats: torch rounds them to float32, as coefs holds
+    torch.add(k[0] * tl, k[1] * I, out=tl)
+    torch.add(k[2] * tm, k[3] * tl, out=tm)
+    contrast = ((tl - tm) / (tm + float(coefs[4]))).clamp(-1.0, float(coefs[5]))
+    contrast = torch.where(torch.from_numpy(has.reshape(-1))[None], contrast, torch.zeros_like(contrast))
+    return contrast, contrast[:, torch.from_numpy(rows)], A, has.reshape(-1)
+
+
+def test_photoreceptor_cells_match_reference_torch_ops_and_float64():
+    """Several photoreceptors per (column, family) cell, empty cells and columns without photoreceptors: the oracle
+    against flyverse/optic.py:496-517 in torch float32 with the reference's averaging matrix (bit for bit), and
+    against a float64 numpy restatement within 1e-5 relative."""
+    rng = np.random.default_rng(11)
+    B, n_col, n_pr = 3, 40, 150
+    pr_col = rng.integers(0, n_col - 3, n_pr)                 # the last three columns have no photoreceptor
+    fam = rng.choice(5, n_pr, p=[0.6, 0.1, 0.1, 0.1, 0.1])
+    retina = Retina(np.arange(n_pr), rng.random((n_pr, 4)).astype(np.float32), pr_col, fam, n_col)
+    cells = retina.cells()
+    assert np.bincount(retina.pr_column * 5 + retina.pr_family).max() >= 5
+    I_lp, I_mean = np.zeros((B, 5 * n_col), np.float32), np.zeros((B, 5 * n_col), np.float32)
+    fresh = np.ones(B, np.uint8)
+    tl, tm = torch.zeros(B, 5 * n_col), torch.zeros(B, 5 * n_col)
+    tfresh = torch.ones(B, dtype=torch.bool)
+    frame_ms = 10.0
+    a_lp, a_ad = float(np.exp(-frame_ms / 10.0)), float(np.exp(-frame_ms / 300.0))
+    coefs = np.array([a_lp, 1 - a_lp, a_ad, 1 - a_ad, 0.02, 2.0], np.float32)
+    d_lp = d_mean = None
+    for frame in range(6):
+        rad = rng.random((B, n_col, 4)).astype(np.float32)
+        con = np.empty((B, 5 * n_col), np.float32)
+        got = oracle.photoreceptor(rad, retina.pr_sens, I_lp, I_mean, fresh, coefs, cells, contrast=con)
+        contrast, a_pr, A, has = _reference_photoreceptor_torch(retina, rad, tl, tm, tfresh, coefs)
+        assert torch.equal(torch.from_numpy(con), contrast) and torch.equal(torch.from_numpy(got), a_pr)
+        assert torch.equal(torch.from_numpy(I_lp), tl) and torch.equal(torch.from_numpy(I_mean), tm)
+        # float64, the same formulas
+        Ip = (rad.astype(np.float64)[:, retina.pr_column] * retina.pr_sens.astype(np.float64)).sum(-1)
+        I = (A.astype(np.float64) @ Ip.T).T
+        if frame == 0:
+            d_lp, d_mean = I.copy(), I.copy()
+        d_lp = a_lp * d_lp + (1 - a_lp) * I
+        d_mean = a_ad * d_mean + (1 - a_ad) * d_lp
+        c64 = np.where(has[None], np.clip((d_lp - d_mean) / (d_mean + 0.02), -1.0, 2.0), 0.0)
+        np.testing.assert_allclose(I_lp, d_lp, rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(con, c64, rtol=1e-5, atol=1e-6)
+        assert (con[:, ~has] == 0).all() and np.array_equal(got, con[:, retina.pr_column * 5 + retina.pr_family])
 
 
 def test_feedback_and_drive_out_and_means():

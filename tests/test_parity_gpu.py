@@ -14,7 +14,7 @@ import oracle
 from flyhip import hip, synth
 from flyhip.brain import Brain, LIFParams
 from flyhip.fly import FlyBrain
-from helpers import fly_setup, oracle_fly, random_csr
+from helpers import fly_setup, grouped_retina, oracle_fly, random_csr
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
@@ -249,22 +249,40 @@ def test_optic_kernels_bit_exact_on_a_side_stream():
             assert_bits(N_(o._cuda_dr), dr, "dr")
 
 
-def test_photoreceptor_feedback_drive_bit_exact():
+@pytest.mark.parametrize("grouped", [False, True])
+def test_photoreceptor_feedback_drive_bit_exact(grouped):
+    """The photoreceptor stage against the oracle: one photoreceptor per column, or several per (column, family)
+    cell with empty cells and columns (the averaging matrix of flyverse/optic.py:320-325)."""
+    from flyhip.connectome import Retina
     rng = np.random.default_rng(8)
     B, n_pr = 3, 777
     sens = rng.random((n_pr, 4)).astype(np.float32)
+    if grouped:
+        retina = Retina(np.arange(n_pr), sens, rng.integers(0, 200, n_pr), rng.choice(5, n_pr, p=[.6, .1, .1, .1, .1]),
+                        205)
+    else:
+        retina = Retina(np.arange(n_pr), sens)
+    cells = retina.cells()
+    tcells = {k: T(v) for k, v in cells.items()}
+    n_col, n_cells = retina.n_col, retina.n_cells
     coefs = np.array([math.exp(-1), 1 - math.exp(-1), math.exp(-10 / 300), 1 - math.exp(-10 / 300), 0.02, 2.0], np.float32)
-    I_lp, I_mean = np.zeros((B, n_pr), np.float32), np.zeros((B, n_pr), np.float32)
+    I_lp, I_mean = np.zeros((B, n_cells), np.float32), np.zeros((B, n_cells), np.float32)
     fresh = np.ones(B, np.uint8)
     tI_lp, tI_mean, tfresh = T(I_lp), T(I_mean), T(fresh)
     a_pr = torch.empty(B, n_pr, device=DEV)
+    contrast = torch.empty(B, n_cells, device=DEV)
     for _ in range(4):
-        rad = rng.random((B, n_pr, 4)).astype(np.float32)
-        hip.photoreceptor(T(rad), T(sens), tI_lp, tI_mean, tfresh, T(coefs), a_pr)
-        ref = oracle.photoreceptor(rad, sens, I_lp, I_mean, fresh, coefs)
-        assert_bits(N_(a_pr), ref, "contrast")
+        rad = rng.random((B, n_col, 4)).astype(np.float32)
+        hip.photoreceptor(T(rad), T(sens), tcells, tI_lp, tI_mean, tfresh, T(coefs), contrast, a_pr)
+        con = np.empty((B, n_cells), np.float32)
+        ref = oracle.photoreceptor(rad, sens, I_lp, I_mean, fresh, coefs, cells, contrast=con)
+        assert_bits(N_(a_pr), ref, "a_pr")
+        assert_bits(N_(contrast), con, "contrast")
         assert_bits(N_(tI_mean), I_mean, "I_mean")
+        assert_bits(N_(tI_lp), I_lp, "I_lp")
     assert int(tfresh.sum()) == 0
+    with pytest.raises(ValueError):
+        hip.photoreceptor(T(rad[:, :-1]), T(sens), tcells, tI_lp, tI_mean, tfresh, T(coefs), contrast, a_pr)
     n = 900
     rate = rng.uniform(0, 400, (B, n)).astype(np.float32)
     spk = np.sort(rng.choice(n, 300, replace=False)).astype(np.int32)
@@ -340,8 +358,11 @@ def test_brain_raster_bit_exact_plain_lif():
 
 @pytest.mark.parametrize("graphs", [False, True])
 def test_fly_frames_bit_exact_with_optic(graphs):
-    """A 2 % scale fly (LIF + optic lobe + photoreceptor movie): 5 frames of 10 ms, rasters identical step for step."""
+    """A 2 % scale fly (LIF + optic lobe + photoreceptor movie): 5 frames of 10 ms, rasters identical step for step.
+    The graph-captured run places its photoreceptors several to a (column, family) cell."""
     conn, retina, layout, lp, op = fly_setup(0.02, seed=1)
+    if graphs:
+        retina = grouped_retina(retina, per_column=4, seed=3)
     B = 2
     fb = FlyBrain(conn, retina, batch=B, device=DEV, seed=9, lif_params=lp, optic_params=op, cuda_graphs=graphs)
     idx, hz = synth.fly_inputs(5, layout, 40)
@@ -356,6 +377,7 @@ def test_fly_frames_bit_exact_with_optic(graphs):
         assert_bits(N_(fb.brain.spikes), raster[-1], f"spikes frame {frame}")
         assert_bits(N_(fb.brain.drive), om.st.drive, f"drive frame {frame}")
         assert_bits(N_(fb.optic.v), om.os.v, f"optic v frame {frame}")
+        assert_bits(N_(fb.optic.I_lp), om.os.I_lp, f"photoreceptor low-pass frame {frame}")
         assert_bits(N_(fb.brain.spike_counts), om.st.spike_counts, f"counts frame {frame}")
     for k in ("v", "g", "rate", "refrac"):
         assert_bits(N_(getattr(fb.brain, k)), getattr(om.st, k), k)
