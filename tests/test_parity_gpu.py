@@ -356,6 +356,32 @@ def test_brain_raster_bit_exact_plain_lif():
         assert total > 0
 
 
+def test_optic_lobe_photoreceptor_activity_and_type_summary():
+    """OpticLobe.photoreceptor_activity (flyverse/optic.py:496-517) on a grouped retina against the oracle, and the
+    per-type summary of the last frame's rate change (optic.py:696-703)."""
+    from flyhip.optic import OpticLobe, photoreceptor_coefs
+    conn, retina, layout, lp, op = fly_setup(0.02, seed=4)
+    retina = grouped_retina(retina, per_column=3, seed=5)
+    B = 2
+    ol = OpticLobe(conn, retina, op, device=DEV, batch=B)
+    assert ol.delta_rate_by_type() is None
+    cells = retina.cells()
+    I_lp = np.zeros((B, retina.n_cells), np.float32)
+    I_mean = np.zeros_like(I_lp)
+    fresh = np.ones(B, np.uint8)
+    rng = np.random.default_rng(6)
+    for frame in range(3):
+        rad = rng.random((B, retina.n_col, 4)).astype(np.float32)
+        got = ol.photoreceptor_activity(T(rad), 10.0)
+        ref = oracle.photoreceptor(rad, retina.pr_sens, I_lp, I_mean, fresh, photoreceptor_coefs(op, 10.0), cells)
+        assert_bits(N_(got), ref, f"activity frame {frame}")
+    assert_bits(N_(ol.I_mean), I_mean, "I_mean")
+    ol.step_frame(T(rad[0]), torch.zeros(B, conn.n, device=DEV), 10.0)
+    summary = ol.delta_rate_by_type(top=5, row=1)
+    assert list(summary.columns) == ["mean_dr", "mean_abs", "n"] and 0 < len(summary) <= 5
+    assert summary["mean_abs"].is_monotonic_decreasing and summary["n"].sum() <= ol.n_rate
+
+
 @pytest.mark.parametrize("graphs", [False, True])
 def test_fly_frames_bit_exact_with_optic(graphs):
     """A 2 % scale fly (LIF + optic lobe + photoreceptor movie): 5 frames of 10 ms, rasters identical step for step.
