@@ -296,15 +296,34 @@ __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
     }
 }
 
-// Backward, part 3 (per edge of W): eg[e] += sum over movies of ly[row_e, b] * dr_t[col_e, b], one thread per edge
-// (all edges in one flat grid, so short rows leave no lanes idle and no row-by-row load chain), the 32 movies in order.
+// Backward, part 3 (per edge of W): eg[e] += sum over movies of ly[row_e, b] * dr_t[col_e, b], the 32 movies in
+// order.  A wavefront takes 64 consecutive edges (all edges in one flat grid, so short rows leave no lanes idle).  It
+// first stages the 64 rows dr_t[col_e, :] in LDS, eight lanes per row with one 16-byte load each, so every load
+// instruction reads 8 whole 128-byte rows instead of 64 scattered pieces; then lane e folds its own edge from LDS,
+// with ly[row_e, :] read directly (consecutive edges share rows).
+constexpr int kEdgeRow = NL + 4;          // LDS row stride in floats: 16-byte aligned rows, staggered banks
+
 __global__ __launch_bounds__(256) void k_adj_back_edges(const int* __restrict__ row_of, const int* __restrict__ idx,
                                                         const float* __restrict__ ly, const float* __restrict__ dr,
                                                         float* __restrict__ eg, int64_t nnz) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float4 stage[4][64 * kEdgeRow / 4];
+    const int lane = threadIdx.x & 63;
+    float4* const buf = stage[threadIdx.x >> 6];
+    const int64_t e0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    if (e0 >= nnz) return;
+    const int part = lane & 7;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int j = i * 8 + (lane >> 3);
+        const int64_t e = min(e0 + j, nnz - 1);
+        const float4 v = ((const float4*)(dr + (size_t)idx[e] * NL))[part];
+        buf[j * (kEdgeRow / 4) + part] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int64_t e = e0 + lane;
     if (e >= nnz) return;
     const float4* l4 = (const float4*)(ly + (size_t)row_of[e] * NL);
-    const float4* d4 = (const float4*)(dr + (size_t)idx[e] * NL);
+    const float4* d4 = buf + lane * (kEdgeRow / 4);
     float s = 0.0f;
 #pragma unroll
     for (int k = 0; k < NL / 4; ++k) {
@@ -430,8 +449,8 @@ FH_EXPORT int launch_adj_back_rows(void** p, int rows, const float* f, int grid,
 FH_EXPORT int launch_adj_back_edges(void** p, int64_t nnz, void* stream) {
     if (nnz < 0) return FH_ERR_ARGS;
     if (!nnz) return 0;
-    hipLaunchKernelGGL(k_adj_back_edges, dim3(nblk(nnz)), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
-                       (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], nnz);
+    hipLaunchKernelGGL(k_adj_back_edges, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const int*)p[0], (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], nnz);
     FH_LAUNCH_CHECK();
 }
 

@@ -95,3 +95,28 @@ def test_adjoint_is_deterministic_and_checkpoint_invariant():
     loss.backward()
     for a, b in zip(th, results[0][1]):
         assert torch.equal(a.grad, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nnz", [1, 63, 64, 200, 4097])
+def test_adjoint_edge_products_bit_exact(nnz):
+    """launch_adj_back_edges: eg[e] += sum over the 32 movies, in movie order, of ly[row_e, b] * dr[col_e, b] (each
+    product rounded), for edge counts around the 64-edge wavefront tiles; torch float32 in the same order."""
+    import ctypes as ct
+    from flyhip.adjoint import _lib
+    rng = np.random.default_rng(nnz)
+    n = 97
+    row_of = np.sort(rng.integers(0, n, nnz)).astype(np.int32)
+    idx = rng.integers(0, n, nnz).astype(np.int32)
+    ly = rng.normal(0, 1, (n, 32)).astype(np.float32)
+    dr = rng.normal(0, 1, (n, 32)).astype(np.float32)
+    eg0 = rng.normal(0, 1, nnz).astype(np.float32)
+    t = [torch.as_tensor(x, device="cuda") for x in (row_of, idx, ly, dr, eg0)]
+    arr = (ct.c_void_p * 5)(*(x.data_ptr() for x in t))
+    assert _lib(t[0].device).launch_adj_back_edges(arr, nnz, torch.cuda.current_stream().cuda_stream) == 0
+    torch.cuda.synchronize()
+    L, D = torch.from_numpy(ly)[torch.from_numpy(row_of).long()], torch.from_numpy(dr)[torch.from_numpy(idx).long()]
+    s = torch.zeros(nnz)
+    for b in range(32):
+        s = s + L[:, b] * D[:, b]
+    assert torch.equal(t[4].cpu(), torch.from_numpy(eg0) + s)
