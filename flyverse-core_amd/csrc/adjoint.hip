@@ -29,6 +29,34 @@ __device__ __forceinline__ float half_sum(float x) {
     return x;
 }
 
+// The 16 sums over the 32 lanes of a half-wave of v[0..15] at once, by a transposing butterfly: at each of four
+// exchange steps a lane keeps one half of its values and adds its partner's copy of them, so the values per lane
+// halve; a last exchange completes the sums.  Lanes 2m and 2m + 1 of the half end with the sum of v[m] (the same
+// fixed tree of additions for every m), for 16 shuffles and 16 adds in all.
+__device__ __forceinline__ float half_sums16(const float (&v)[16], int l) {
+    float a8[8], a4[4], a2[2];
+    const bool u16 = l & 16, u8 = l & 8, u4 = l & 4, u2 = l & 2;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float keep = u16 ? v[k + 8] : v[k], give = u16 ? v[k] : v[k + 8];
+        a8[k] = keep + __shfl_xor(give, 16, 32);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float keep = u8 ? a8[k + 4] : a8[k], give = u8 ? a8[k] : a8[k + 4];
+        a4[k] = keep + __shfl_xor(give, 8, 32);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const float keep = u4 ? a4[k + 2] : a4[k], give = u4 ? a4[k] : a4[k + 2];
+        a2[k] = keep + __shfl_xor(give, 4, 32);
+    }
+    const float keep = u2 ? a2[1] : a2[0], give = u2 ? a2[0] : a2[1];
+    float a1 = keep + __shfl_xor(give, 2, 32);
+    a1 = a1 + __shfl_xor(a1, 1, 32);
+    return a1;
+}
+
 typedef const __attribute__((address_space(4))) int* ci32p;
 typedef const __attribute__((address_space(4))) float* cf32p;
 
@@ -97,9 +125,13 @@ struct HalfBuf {
     float x[16];
 };
 
+// With EG, each chunk also adds to eg[16 c + m] (the chunk-padded order of this CSR) the sum over the half's 32 lanes
+// of x[i_m, l] * own[l], own being the half's own row of another (rows, 32) array: the per-edge products of the
+// transposed product's gathers with the row's own values (half_sums16), written once per real entry.
+template <bool EG>
 __device__ __forceinline__ float fold32_pair_padded(const int4* __restrict__ pidx, const float4* __restrict__ pw,
                                                     const float* __restrict__ x, int ca0, int ca1, int cb0, int cb1,
-                                                    int l, bool h) {
+                                                    int l, bool h, float own, float* __restrict__ eg) {
     const int n0 = ca1 - ca0, n1 = cb1 - cb0, nmax = max(n0, n1);
     const int c0 = h ? cb0 : ca0, nh = h ? n1 : n0;
     auto load = [&](int k, HalfBuf& b) {
@@ -115,12 +147,27 @@ __device__ __forceinline__ float fold32_pair_padded(const int4* __restrict__ pid
         for (int j = 0; j < 16; ++j) b.x[j] = x[(size_t)max(b.i[j], 0) * NL + l];
     };
     float total = 0.0f;
-    auto fold = [&](int k, const HalfBuf& b) {
+    auto fold = [&](int k, HalfBuf& b) {
         float s = 0.0f;
 #pragma unroll
         for (int j = 0; j < 16; ++j)
             if (b.i[j] >= 0) { const float p = b.w[j] * b.x[j]; s = s + p; }
         if (k < nh) total = total + s;
+        if (EG) {
+            // the products with the row's own values, in place of the gathered values (pads contribute 0)
+            uint32_t real = 0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                b.x[j] = b.i[j] >= 0 ? b.x[j] * own : 0.0f;
+                real |= (b.i[j] >= 0 ? 1u : 0u) << j;
+            }
+            const float e = half_sums16(b.x, l);
+            const int m = (l >> 1) & 15;
+            if (k < nh && (l & 1) == 0 && ((real >> m) & 1u)) {
+                float* q = eg + (int64_t)(c0 + k) * 16 + m;
+                *q = *q + e;
+            }
+        }
     };
     if (nmax == 0) return 0.0f;
     HalfBuf A, B;
@@ -183,8 +230,9 @@ __global__ __launch_bounds__(256) void k_adj_forward(FwdArgs f) {
         const int r0 = f.sched[2 * pr];
         const int r1 = 2 * pr + 1 < f.rows ? f.sched[2 * pr + 1] : -1;
         const int r = h ? r1 : r0;
-        const float y = fold32_pair_padded((const int4*)f.idx, (const float4*)f.w, f.dr, f.ptr[r0], f.ptr[r0 + 1],
-                                           r1 < 0 ? 0 : f.ptr[r1], r1 < 0 ? 0 : f.ptr[r1 + 1], l, h);
+        const float y = fold32_pair_padded<false>((const int4*)f.idx, (const float4*)f.w, f.dr, f.ptr[r0],
+                                                  f.ptr[r0 + 1], r1 < 0 ? 0 : f.ptr[r1], r1 < 0 ? 0 : f.ptr[r1 + 1],
+                                                  l, h, 0.0f, nullptr);
         if (r < 0) continue;
         const int64_t q = (int64_t)r * NL + l;
         float inp = f.g_rr * y;
@@ -244,6 +292,7 @@ struct BackArgs {
     const int* readout;
     const float* target;                 // (R, 32) targets of step t, or null (t = 0)
     float* gb;                           // (n) per-unit bias gradient accumulator
+    float* eg;                           // per-edge gain partials in chunk-padded W^T order, or null
     int rows;
     float g_ad, a_ad, loss_scale;
 };
@@ -253,6 +302,9 @@ struct BackArgs {
 //   linp = lv_t+1 * (1 - a_j);  ladapt_t = la_t+1 * a_ad - g_ad * linp
 //   gate = [0 <= v_t + b_j <= 1] (torch clamp convention); lr = loss_scale * (r_t - target_t) on readout units
 //   lv_t = lv_t+1 * a_j + (ldr + lr) * gate;  gb_j += sum_b ldr * (gate - 1) + lr * gate
+// With EG the same gathers also give the per-edge gain partials: eg (chunk-padded W^T order) += sum over movies of
+// ly[row, b] * dr_t[j, b] for each entry (row) of unit j's column.
+template <bool EG>
 __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
     const int l = threadIdx.x & (NL - 1);
     const int pairs = (k.rows + 1) / 2;
@@ -261,8 +313,10 @@ __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
         const int j0 = k.schedT[2 * pr];
         const int j1 = 2 * pr + 1 < k.rows ? k.schedT[2 * pr + 1] : -1;
         const int j = h ? j1 : j0;
-        const float s = fold32_pair_padded((const int4*)k.idxT, (const float4*)k.wT, k.ly, k.ptrT[j0],
-                                           k.ptrT[j0 + 1], j1 < 0 ? 0 : k.ptrT[j1], j1 < 0 ? 0 : k.ptrT[j1 + 1], l, h);
+        const float own = EG && j >= 0 ? k.dr[(int64_t)j * NL + l] : 0.0f;
+        const float s = fold32_pair_padded<EG>((const int4*)k.idxT, (const float4*)k.wT, k.ly, k.ptrT[j0],
+                                               k.ptrT[j0 + 1], j1 < 0 ? 0 : k.ptrT[j1], j1 < 0 ? 0 : k.ptrT[j1 + 1],
+                                               l, h, own, k.eg);
         if (j < 0) continue;
         const int64_t q = (int64_t)j * NL + l;
         const float la1 = k.la[q], lv1 = k.lv[q];
@@ -294,44 +348,6 @@ __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
         gbb = half_sum(gbb);
         if (l == 0) k.gb[j] = k.gb[j] + gbb;
     }
-}
-
-// Backward, part 3 (per edge of W): eg[e] += sum over movies of ly[row_e, b] * dr_t[col_e, b], the 32 movies in
-// order.  A wavefront takes 64 consecutive edges (all edges in one flat grid, so short rows leave no lanes idle).  It
-// first stages the 64 rows dr_t[col_e, :] in LDS, eight lanes per row with one 16-byte load each, so every load
-// instruction reads 8 whole 128-byte rows instead of 64 scattered pieces; then lane e folds its own edge from LDS,
-// with ly[row_e, :] read directly (consecutive edges share rows).
-constexpr int kEdgeRow = NL + 4;          // LDS row stride in floats: 16-byte aligned rows, staggered banks
-
-__global__ __launch_bounds__(256) void k_adj_back_edges(const int* __restrict__ row_of, const int* __restrict__ idx,
-                                                        const float* __restrict__ ly, const float* __restrict__ dr,
-                                                        float* __restrict__ eg, int64_t nnz) {
-    __shared__ float4 stage[4][64 * kEdgeRow / 4];
-    const int lane = threadIdx.x & 63;
-    float4* const buf = stage[threadIdx.x >> 6];
-    const int64_t e0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-    if (e0 >= nnz) return;
-    const int part = lane & 7;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int j = i * 8 + (lane >> 3);
-        const int64_t e = min(e0 + j, nnz - 1);
-        const float4 v = ((const float4*)(dr + (size_t)idx[e] * NL))[part];
-        buf[j * (kEdgeRow / 4) + part] = v;
-    }
-    __builtin_amdgcn_wave_barrier();
-    const int64_t e = e0 + lane;
-    if (e >= nnz) return;
-    const float4* l4 = (const float4*)(ly + (size_t)row_of[e] * NL);
-    const float4* d4 = buf + lane * (kEdgeRow / 4);
-    float s = 0.0f;
-#pragma unroll
-    for (int k = 0; k < NL / 4; ++k) {
-        const float4 a = l4[k], t = d4[k];
-        float p0 = a.x * t.x, p1 = a.y * t.y, p2 = a.z * t.z, p3 = a.w * t.w;
-        s = s + p0; s = s + p1; s = s + p2; s = s + p3;
-    }
-    eg[e] = eg[e] + s;
 }
 
 // out[k] = scale_k * sum over the segment [seg[k], seg[k+1]) of vals[perm[m]] * (mult ? mult[perm[m]] : 1):
@@ -430,7 +446,7 @@ FH_EXPORT int launch_adj_back_ly(void** p, int rows, float g_rr, void* stream) {
 }
 
 // p = [ptrT, idxT, schedT, wT (chunk-padded W^T as launch_adj_forward's W), a, b, ly, v, dr, lv, la, readout,
-// target|null, gb]; f = [g_ad, a_ad, loss_scale].
+// target|null, gb, eg|null (16 per chunk of W^T)]; f = [g_ad, a_ad, loss_scale].
 FH_EXPORT int launch_adj_back_rows(void** p, int rows, const float* f, int grid, void* stream) {
     if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
     if (!rows) return 0;
@@ -439,18 +455,12 @@ FH_EXPORT int launch_adj_back_rows(void** p, int rows, const float* f, int grid,
     k.wT = (const float*)p[3]; k.a = (const float*)p[4]; k.b = (const float*)p[5];
     k.ly = (const float*)p[6]; k.v = (const float*)p[7]; k.dr = (const float*)p[8];
     k.lv = (float*)p[9]; k.la = (float*)p[10]; k.readout = (const int*)p[11]; k.target = (const float*)p[12];
-    k.gb = (float*)p[13];
+    k.gb = (float*)p[13]; k.eg = (float*)p[14];
     k.rows = rows; k.g_ad = f[0]; k.a_ad = f[1]; k.loss_scale = f[2];
-    hipLaunchKernelGGL(k_adj_back_rows, dim3(grid), dim3(256), 0, (hipStream_t)stream, k);
-    FH_LAUNCH_CHECK();
-}
-
-// p = [row_of (nnz: the row of each edge), idx (nnz), ly (n, 32), dr (n, 32), eg (nnz)]
-FH_EXPORT int launch_adj_back_edges(void** p, int64_t nnz, void* stream) {
-    if (nnz < 0) return FH_ERR_ARGS;
-    if (!nnz) return 0;
-    hipLaunchKernelGGL(k_adj_back_edges, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       (const int*)p[0], (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], nnz);
+    if (k.eg)
+        hipLaunchKernelGGL(k_adj_back_rows<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, k);
+    else
+        hipLaunchKernelGGL(k_adj_back_rows<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream, k);
     FH_LAUNCH_CHECK();
 }
 

@@ -40,7 +40,6 @@ SIGS = {
     "launch_adj_forward": [_P, _I, _FP, _I, _S],
     "launch_adj_back_ly": [_P, _I, _F, _S],
     "launch_adj_back_rows": [_P, _I, _FP, _I, _S],
-    "launch_adj_back_edges": [_P, ct.c_int64, _S],
     "launch_segment_sum": [_P, _I, _S],
     "launch_loss_total": [_P, _L, _F, _S],
 }
@@ -102,8 +101,6 @@ class OpticAdjoint:
             raise ValueError("checkpoint must be >= 1")
         t = lambda a, dt=None: torch.as_tensor(np.ascontiguousarray(a), device=dev, dtype=dt)
         rows = np.repeat(np.arange(n), np.diff(W0.indptr))
-        self.row_of, self.idx = t(rows.astype(np.int32)), t(W0.indices.astype(np.int32))
-        self.w0 = t(W0.data)
         bins = (ut[rows] * K + ut[W0.indices]).astype(np.int32)
         self.sched = t(np.argsort(-np.diff(W0.indptr), kind="stable").astype(np.int32))
         # the forward product's chunk-padded copy (hip.chunk_padded): weights of pads are 0 * gain
@@ -122,8 +119,10 @@ class OpticAdjoint:
         self.bptr, self.bidx, self.bw0, self.bbin = t(bptr.astype(np.int32)), t(bidx), t(bw0), t(bbin)
         self.schedT = t(np.argsort(-np.diff(WT.indptr), kind="stable").astype(np.int32))
         self.nnz = W0.nnz
-        pg, sg = _segments(bins.astype(np.int64), K * K)
-        self.permG, self.segG = t(pg), t(sg)
+        # the per-edge gain partials accumulate in the transposed product's chunk-padded order (its pads never read)
+        realT = np.flatnonzero(bidx >= 0)
+        pg, sg = _segments(bbin[realT].astype(np.int64), K * K)
+        self.permG, self.segG = t(realT[pg].astype(np.int32)), t(sg)
         pt, st = _segments(ut, K)
         self.permT, self.segT = t(pt), t(st)
         self.unit_type = t(ut.astype(np.int32))
@@ -147,7 +146,7 @@ class OpticAdjoint:
         C = self.checkpoint
         self.S_v, self.S_a, self.S_dr, self.S_inp = z(C + 1, n, NL), z(C + 1, n, NL), z(C + 1, n, NL), z(C, n, NL)
         self.lv, self.la, self.ly = z(n, NL), z(n, NL), z(n, NL)
-        self.ga, self.gb, self.eg = z(n), z(n), z(self.nnz)
+        self.ga, self.gb, self.eg = z(n), z(n), z(len(bidx))
         self._params = None
         self._last_segment_ready = False
 
@@ -258,24 +257,25 @@ class OpticAdjoint:
                 t = t0 + k
                 self._call("launch_adj_back_ly", [self.lv, self.S_v[k], self.S_inp[k], self.a, self.ly, self.ga], n,
                            float(self.c.gain_rr))
-                self._back_rows(k, t, f_back)
-                self._call("launch_adj_back_edges", [self.row_of, self.idx, self.ly, self.S_dr[k], self.eg], self.nnz)
+                self._back_rows(k, t, f_back, self.eg)
         tau, bias, gain = self._params
         K = self.K
         ga_t, gb_t, gg = (torch.empty(K, device=self.device), torch.empty(K, device=self.device),
                           torch.empty(K * K, device=self.device))
         self._call("launch_segment_sum", [self.ga, None, self.permT, self.segT, ga_t], K)
         self._call("launch_segment_sum", [self.gb, None, self.permT, self.segT, gb_t], K)
-        self._call("launch_segment_sum", [self.eg, self.w0, self.permG, self.segG, gg], K * K)
+        self._call("launch_segment_sum", [self.eg, self.bw0, self.permG, self.segG, gg], K * K)
         a_k = torch.exp(-self.c.dt / tau)
         g_tau = ga_t * a_k * (self.c.dt / (tau * tau))
         return g_tau, gb_t, gg.view(K, K)
 
-    def _back_rows(self, k, t, f_back):
+    def _back_rows(self, k, t, f_back, eg=None):
+        """The transposed product and the rest of step t's adjoint; with eg, the same gathers also add each edge's
+        sum over movies of ly[row] * dr_t[col] to eg (chunk-padded W^T order)."""
         target = self.target[t - 1] if t >= 1 else None
         self._call("launch_adj_back_rows",
                    [self.bptr, self.bidx, self.schedT, self.bw, self.a, self.b, self.ly, self.S_v[k], self.S_dr[k],
-                    self.lv, self.la, self.readout, target, self.gb],
+                    self.lv, self.la, self.readout, target, self.gb, eg],
                    self.n, f_back, self.grid)
 
 

@@ -204,12 +204,9 @@ int launch_adj_forward(void** p, int rows, const float* f, int grid, void* strea
 int launch_adj_back_ly(void** p, int rows, float g_rr, void* stream);
 
 /* Adjoint of one step through the transpose CSR (W^T ly) and the clip gates.  p = [ptrT, idxT, schedT, wT, a, b, ly,
- * v, dr, lv, la, readout, target|null, gb]; f = [g_ad, a_ad, loss_scale]. */
+ * v, dr, lv, la, readout, target|null, gb, eg|null]; f = [g_ad, a_ad, loss_scale].  With eg (16 floats per chunk of
+ * the chunk-padded W^T) each real entry also gets eg += sum over movies of ly[row] * dr[col], from the same gathers. */
 int launch_adj_back_rows(void** p, int rows, const float* f, int grid, void* stream);
-
-/* Per-edge gain partials eg[e] += sum over movies of ly[row] * dr[col], one thread per edge.
- * p = [row_of (nnz: row of each edge), idx (nnz), ly (n, 32), dr (n, 32), eg (nnz)]. */
-int launch_adj_back_edges(void** p, int64_t nnz, void* stream);
 
 /* out[s] = fixed-order sum over segment s of vals[perm[j]] (* mult[perm[j]]).  p = [vals, mult|null, perm,
  * seg (nseg+1), out]. */

@@ -97,26 +97,55 @@ def test_adjoint_is_deterministic_and_checkpoint_invariant():
         assert torch.equal(a.grad, b)
 
 
+def _half_sums16(p):
+    """The per-entry sums over 32 lanes of the adjoint's transposing butterfly, on the CPU: p (..., 32 lanes, 16
+    entries) float32 -> (..., 16), each entry the value lane 2m ends with (float32 adds in the kernel's tree)."""
+    lanes = np.arange(32)
+    vals = p.astype(np.float32)
+    for bit, half in ((16, 8), (8, 4), (4, 2), (2, 1)):
+        up = (lanes & bit) != 0
+        keep = np.where(up[:, None], vals[..., half:2 * half], vals[..., :half])
+        give = np.where(up[:, None], vals[..., :half], vals[..., half:2 * half])
+        vals = (keep + give[..., lanes ^ bit, :]).astype(np.float32)
+    v = vals[..., 0]
+    v = (v + v[..., lanes ^ 1]).astype(np.float32)
+    return v[..., 0::2]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("nnz", [1, 63, 64, 200, 4097])
-def test_adjoint_edge_products_bit_exact(nnz):
-    """launch_adj_back_edges: eg[e] += sum over the 32 movies, in movie order, of ly[row_e, b] * dr[col_e, b] (each
-    product rounded), for edge counts around the 64-edge wavefront tiles; torch float32 in the same order."""
+def test_adjoint_edge_partials_from_the_transposed_product():
+    """The per-edge gain partials that launch_adj_back_rows adds from its own gathers: for each entry (W row r) of
+    unit j's column, eg += sum over the 32 movies of ly[r, b] * dr[j, b] in the kernel's fixed butterfly order (bit
+    for bit against a CPU emulation), within 1e-5 of float64; pads and a null eg are left alone."""
     import ctypes as ct
-    from flyhip.adjoint import _lib
-    rng = np.random.default_rng(nnz)
-    n = 97
-    row_of = np.sort(rng.integers(0, n, nnz)).astype(np.int32)
-    idx = rng.integers(0, n, nnz).astype(np.int32)
-    ly = rng.normal(0, 1, (n, 32)).astype(np.float32)
-    dr = rng.normal(0, 1, (n, 32)).astype(np.float32)
-    eg0 = rng.normal(0, 1, nnz).astype(np.float32)
-    t = [torch.as_tensor(x, device="cuda") for x in (row_of, idx, ly, dr, eg0)]
-    arr = (ct.c_void_p * 5)(*(x.data_ptr() for x in t))
-    assert _lib(t[0].device).launch_adj_back_edges(arr, nnz, torch.cuda.current_stream().cuda_stream) == 0
+    from flyhip.adjoint import OpticAdjoint
+    W0, ut, P, target, readout, tau, bias, gain = small_problem(n=200, nnz=9000, seed=4)
+    prob = OpticAdjoint(W0, ut, 5, P, target, readout, device="cuda")
+    t = lambda x: torch.as_tensor(x, device="cuda")
+    prob.loss(t(tau), t(bias), t(gain))
+    rng = np.random.default_rng(2)
+    ly = rng.normal(0, 1, (prob.n, 32)).astype(np.float32)
+    dr = rng.normal(0, 1, (prob.n, 32)).astype(np.float32)
+    prob.ly.copy_(t(ly))
+    prob.S_dr[0].copy_(t(dr))
+    eg0 = rng.normal(0, 1, prob.eg.numel()).astype(np.float32)
+    eg = t(eg0.copy())
+    f_back = (ct.c_float * 3)(1.0, prob.a_ad, 0.01)
+    prob._back_rows(0, 1, f_back, eg)
     torch.cuda.synchronize()
-    L, D = torch.from_numpy(ly)[torch.from_numpy(row_of).long()], torch.from_numpy(dr)[torch.from_numpy(idx).long()]
-    s = torch.zeros(nnz)
-    for b in range(32):
-        s = s + L[:, b] * D[:, b]
-    assert torch.equal(t[4].cpu(), torch.from_numpy(eg0) + s)
+    got = eg.cpu().numpy()
+    bptr, bidx = prob.bptr.cpu().numpy(), prob.bidx.cpu().numpy()
+    unit = np.repeat(np.arange(prob.n), np.diff(bptr))                 # the W^T row (unit) of each chunk
+    chunks = bidx.reshape(-1, 16)
+    prods = (ly[np.maximum(chunks, 0)] * dr[unit][:, None, :]).astype(np.float32)    # (chunks, 16, 32)
+    prods[chunks < 0] = 0.0
+    sums = _half_sums16(np.transpose(prods, (0, 2, 1)))                 # (chunks, 16)
+    want = eg0.reshape(-1, 16).copy()
+    real = chunks >= 0
+    want[real] = (want[real] + sums[real]).astype(np.float32)
+    assert np.array_equal(got, want.reshape(-1))
+    exact = (ly[np.maximum(chunks, 0)].astype(np.float64) * dr[unit][:, None, :]).sum(-1)
+    np.testing.assert_allclose(sums[real], exact[real], rtol=1e-5, atol=1e-5)
+    before = prob.lv.clone()
+    prob._back_rows(0, 1, f_back, None)                                 # no eg: the step's other outputs only
+    assert not torch.equal(before, prob.lv)
