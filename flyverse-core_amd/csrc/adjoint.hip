@@ -15,6 +15,8 @@
 // forward and backward are deterministic and a recomputed segment reproduces the first forward bit for bit.
 #include "common.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int NL = 32;   // movies per row: one half-wavefront
@@ -115,70 +117,100 @@ __device__ __forceinline__ float fold32_pair(ci32p idx, cf32p w, const float* __
 }
 
 // fold32_pair over chunk-padded storage (row r's chunks [cptr[r], cptr[r+1]), chunk c = entries [16 c, 16 c + 16) of
-// pidx / pw, pads idx -1 and skipped): each half-wave loads its own row's chunk with 16-byte loads (the same
-// addresses on all its lanes) and gathers its 16 rows of x; the loads of chunk k + 1 are issued before chunk k is
-// folded (two buffers alternating).  Past a row's last chunk a half repeats that chunk's loads and folds nothing.
-// Same additions, same order as fold32_pair.
-struct HalfBuf {
-    int i[16];
-    float w[16];
-    float x[16];
-};
-
+// pidx / pw, pads idx -1 and skipped), with the same additions in the same order.  Per chunk one vector load brings
+// both halves' indices and weights (in each half, lanes 0-15 the indices, lanes 16-31 the weights); each half reads
+// its own through lane swizzles and gathers its 16 rows of x.  The stream is three chunks deep: the index/weight load of
+// chunk k + 2 and the gathers of chunk k + 1 are in flight while chunk k is folded (vector loads complete in issue
+// order, so each wait leaves the younger loads outstanding).  Past a row's last chunk a half repeats that chunk's
+// loads and folds nothing.
+//
 // With EG, each chunk also adds to eg[16 c + m] (the chunk-padded order of this CSR) the sum over the half's 32 lanes
 // of x[i_m, l] * own[l], own being the half's own row of another (rows, 32) array: the per-edge products of the
 // transposed product's gathers with the row's own values (half_sums16), written once per real entry.
+// Lane J of the caller's own half-wavefront: ds_swizzle in bit mode, and-mask 0, or-mask J.
+template <int J>
+__device__ __forceinline__ int half_lane(int v) {
+    return __builtin_amdgcn_ds_swizzle(v, J << 5);
+}
+
+// f(std::integral_constant<int, J>) for J = J0 .. 15, so J can be a constant expression inside f.
+template <int J, class F>
+__device__ __forceinline__ void unroll16(F&& f) {
+    if constexpr (J < 16) {
+        f(std::integral_constant<int, J>{});
+        unroll16<J + 1>(f);
+    }
+}
+
+struct HalfStage {
+    float x[16];
+    float w[16];
+    uint32_t real;                       // bit j: entry j of this half's chunk is real (not a pad)
+};
+
 template <bool EG>
-__device__ __forceinline__ float fold32_pair_padded(const int4* __restrict__ pidx, const float4* __restrict__ pw,
+__device__ __forceinline__ float fold32_pair_padded(const int* __restrict__ pidx, const float* __restrict__ pw,
                                                     const float* __restrict__ x, int ca0, int ca1, int cb0, int cb1,
                                                     int l, bool h, float own, float* __restrict__ eg) {
     const int n0 = ca1 - ca0, n1 = cb1 - cb0, nmax = max(n0, n1);
+    if (nmax == 0) return 0.0f;
     const int c0 = h ? cb0 : ca0, nh = h ? n1 : n0;
-    auto load = [&](int k, HalfBuf& b) {
+    auto load_v = [&](int k) -> uint32_t {
         const int c = max(c0 + min(k, nh - 1), 0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int4 iv = pidx[(int64_t)c * 4 + q];
-            const float4 wv = pw[(int64_t)c * 4 + q];
-            b.i[4 * q] = iv.x; b.i[4 * q + 1] = iv.y; b.i[4 * q + 2] = iv.z; b.i[4 * q + 3] = iv.w;
-            b.w[4 * q] = wv.x; b.w[4 * q + 1] = wv.y; b.w[4 * q + 2] = wv.z; b.w[4 * q + 3] = wv.w;
-        }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) b.x[j] = x[(size_t)max(b.i[j], 0) * NL + l];
+        const uint32_t* src = l < 16 ? (const uint32_t*)pidx + (size_t)c * 16 + l
+                                     : (const uint32_t*)pw + (size_t)c * 16 + (l - 16);
+        return *src;
     };
+    auto issue = [&](uint32_t V, HalfStage& st) {
+        st.real = 0;
+        unroll16<0>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const int ij = half_lane<j>((int)V);
+            st.w[j] = __int_as_float(half_lane<16 + j>((int)V));
+            st.real |= (ij >= 0 ? 1u : 0u) << j;
+            st.x[j] = x[(size_t)max(ij, 0) * NL + l];
+        });
+    };
+    // the barriers keep the issue order load_v, gathers, fold (the waits depend on it)
+    auto pin = [] { __asm__ volatile("" ::: "memory"); __builtin_amdgcn_sched_barrier(0); };
     float total = 0.0f;
-    auto fold = [&](int k, HalfBuf& b) {
+    auto fold = [&](int k, HalfStage& b) {
         float s = 0.0f;
 #pragma unroll
         for (int j = 0; j < 16; ++j)
-            if (b.i[j] >= 0) { const float p = b.w[j] * b.x[j]; s = s + p; }
+            if ((b.real >> j) & 1u) { const float p = b.w[j] * b.x[j]; s = s + p; }
         if (k < nh) total = total + s;
         if (EG) {
             // the products with the row's own values, in place of the gathered values (pads contribute 0)
-            uint32_t real = 0;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                b.x[j] = b.i[j] >= 0 ? b.x[j] * own : 0.0f;
-                real |= (b.i[j] >= 0 ? 1u : 0u) << j;
-            }
+            for (int j = 0; j < 16; ++j) b.x[j] = ((b.real >> j) & 1u) ? b.x[j] * own : 0.0f;
             const float e = half_sums16(b.x, l);
             const int m = (l >> 1) & 15;
-            if (k < nh && (l & 1) == 0 && ((real >> m) & 1u)) {
+            if (k < nh && (l & 1) == 0 && ((b.real >> m) & 1u)) {
                 float* q = eg + (int64_t)(c0 + k) * 16 + m;
                 *q = *q + e;
             }
         }
     };
-    if (nmax == 0) return 0.0f;
-    HalfBuf A, B;
-    load(0, A);
-    for (int k = 0; k < nmax; k += 2) {
-        load(k + 1, B);
-        fold(k, A);
-        if (k + 1 < nmax) {
-            load(k + 2, A);
-            fold(k + 1, B);
-        }
+    HalfStage SA, SB;
+    const uint32_t V0 = load_v(0);
+    uint32_t VA, VB = load_v(1);
+    pin();
+    issue(V0, SA);
+    pin();
+    for (int k = 0;;) {
+        VA = load_v(k + 2);
+        pin();
+        issue(VB, SB);
+        pin();
+        fold(k, SA);
+        if (++k >= nmax) break;
+        VB = load_v(k + 2);
+        pin();
+        issue(VA, SA);
+        pin();
+        fold(k, SB);
+        if (++k >= nmax) break;
     }
     return total;
 }
@@ -230,7 +262,7 @@ __global__ __launch_bounds__(256) void k_adj_forward(FwdArgs f) {
         const int r0 = f.sched[2 * pr];
         const int r1 = 2 * pr + 1 < f.rows ? f.sched[2 * pr + 1] : -1;
         const int r = h ? r1 : r0;
-        const float y = fold32_pair_padded<false>((const int4*)f.idx, (const float4*)f.w, f.dr, f.ptr[r0],
+        const float y = fold32_pair_padded<false>(f.idx, f.w, f.dr, f.ptr[r0],
                                                   f.ptr[r0 + 1], r1 < 0 ? 0 : f.ptr[r1], r1 < 0 ? 0 : f.ptr[r1 + 1],
                                                   l, h, 0.0f, nullptr);
         if (r < 0) continue;
@@ -314,7 +346,7 @@ __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
         const int j1 = 2 * pr + 1 < k.rows ? k.schedT[2 * pr + 1] : -1;
         const int j = h ? j1 : j0;
         const float own = EG && j >= 0 ? k.dr[(int64_t)j * NL + l] : 0.0f;
-        const float s = fold32_pair_padded<EG>((const int4*)k.idxT, (const float4*)k.wT, k.ly, k.ptrT[j0],
+        const float s = fold32_pair_padded<EG>(k.idxT, k.wT, k.ly, k.ptrT[j0],
                                                k.ptrT[j0 + 1], j1 < 0 ? 0 : k.ptrT[j1], j1 < 0 ? 0 : k.ptrT[j1 + 1],
                                                l, h, own, k.eg);
         if (j < 0) continue;
