@@ -701,10 +701,11 @@ __global__ __launch_bounds__(256) void k_photoreceptor_fm(const float* __restric
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n_cells * 64) return;
     const int lane = (int)(t & 63), k = (int)(t >> 6);
-    if (!((*live_mask >> lane) & 1ull)) return;
+    const int e0 = cell_ptr[k], e1 = cell_ptr[k + 1];
+    // a cell without photoreceptors feeds nothing: its state stays at the zeros it starts from (its input is 0)
+    if (e0 == e1 || !((*live_mask >> lane) & 1ull)) return;
     const uint64_t ms = movie_seed[lane];
     const uint32_t fr = frame[lane];
-    const int e0 = cell_ptr[k], e1 = cell_ptr[k + 1];
     float I = 0.0f;
     for (int cs = e0; cs < e1; cs += 16) {
         const int ce = min(cs + 16, e1);
