@@ -116,17 +116,6 @@ __device__ __forceinline__ float fold32_pair(ci32p idx, cf32p w, const float* __
     return total;
 }
 
-// fold32_pair over chunk-padded storage (row r's chunks [cptr[r], cptr[r+1]), chunk c = entries [16 c, 16 c + 16) of
-// pidx / pw, pads idx -1 and skipped), with the same additions in the same order.  Per chunk one vector load brings
-// both halves' indices and weights (in each half, lanes 0-15 the indices, lanes 16-31 the weights); each half reads
-// its own through lane swizzles and gathers its 16 rows of x.  The stream is three chunks deep: the index/weight load of
-// chunk k + 2 and the gathers of chunk k + 1 are in flight while chunk k is folded (vector loads complete in issue
-// order, so each wait leaves the younger loads outstanding).  Past a row's last chunk a half repeats that chunk's
-// loads and folds nothing.
-//
-// With EG, each chunk also adds to eg[16 c + m] (the chunk-padded order of this CSR) the sum over the half's 32 lanes
-// of x[i_m, l] * own[l], own being the half's own row of another (rows, 32) array: the per-edge products of the
-// transposed product's gathers with the row's own values (half_sums16), written once per real entry.
 // Lane J of the caller's own half-wavefront: ds_swizzle in bit mode, and-mask 0, or-mask J.
 template <int J>
 __device__ __forceinline__ int half_lane(int v) {
@@ -142,9 +131,19 @@ __device__ __forceinline__ void unroll16(F&& f) {
     }
 }
 
+// fold32_pair over chunk-padded storage (row r's chunks [cptr[r], cptr[r+1]), chunk c = entries [16 c, 16 c + 16) of
+// pidx / pw, pads idx -1 and skipped), with the same additions in the same order.  Per chunk one vector load brings
+// both halves' indices and weights (in each half, lanes 0-15 the indices, lanes 16-31 the weights); each half reads
+// its own through lane swizzles, the indices when it gathers its 16 rows of x, the weights when it folds them.  The
+// stream is three chunks deep: the index/weight load of chunk k + 2 and the gathers of chunk k + 1 are in flight
+// while chunk k is folded (vector loads complete in issue order, so each wait leaves the younger loads
+// outstanding).  Past a row's last chunk a half repeats that chunk's loads and folds nothing.
+//
+// With EG, each chunk also adds to eg[16 c + m] (the chunk-padded order of this CSR) the sum over the half's 32 lanes
+// of x[i_m, l] * own[l], own being the half's own row of another (rows, 32) array: the per-edge products of the
+// transposed product's gathers with the row's own values (half_sums16), written once per real entry.
 struct HalfStage {
     float x[16];
-    float w[16];
     uint32_t real;                       // bit j: entry j of this half's chunk is real (not a pad)
 };
 
@@ -166,7 +165,6 @@ __device__ __forceinline__ float fold32_pair_padded(const int* __restrict__ pidx
         unroll16<0>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
             const int ij = half_lane<j>((int)V);
-            st.w[j] = __int_as_float(half_lane<16 + j>((int)V));
             st.real |= (ij >= 0 ? 1u : 0u) << j;
             st.x[j] = x[(size_t)max(ij, 0) * NL + l];
         });
@@ -174,11 +172,14 @@ __device__ __forceinline__ float fold32_pair_padded(const int* __restrict__ pidx
     // the barriers keep the issue order load_v, gathers, fold (the waits depend on it)
     auto pin = [] { __asm__ volatile("" ::: "memory"); __builtin_amdgcn_sched_barrier(0); };
     float total = 0.0f;
-    auto fold = [&](int k, HalfStage& b) {
+    // chunk k from its gathered rows and its index/weight word V (the weights are read out only here)
+    auto fold = [&](int k, HalfStage& b, uint32_t V) {
         float s = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 16; ++j)
-            if ((b.real >> j) & 1u) { const float p = b.w[j] * b.x[j]; s = s + p; }
+        unroll16<0>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const float w = __int_as_float(half_lane<16 + j>((int)V));
+            if ((b.real >> j) & 1u) { const float p = w * b.x[j]; s = s + p; }
+        });
         if (k < nh) total = total + s;
         if (EG) {
             // the products with the row's own values, in place of the gathered values (pads contribute 0)
@@ -192,24 +193,32 @@ __device__ __forceinline__ float fold32_pair_padded(const int* __restrict__ pidx
             }
         }
     };
-    HalfStage SA, SB;
-    const uint32_t V0 = load_v(0);
-    uint32_t VA, VB = load_v(1);
+    // iteration k: the word of chunk k + 2 is loaded, chunk k + 1 gathered, chunk k folded.  Words rotate through
+    // three registers and stages through two, so no register that a load is still writing is read or reused.
+    HalfStage S0, S1;
+    uint32_t V0 = load_v(0), V1 = load_v(1), V2;
     pin();
-    issue(V0, SA);
+    issue(V0, S0);
     pin();
+    auto step = [&](int k, uint32_t& vload, uint32_t vissue, HalfStage& sissue, HalfStage& sfold, uint32_t vfold) {
+        vload = load_v(k + 2);
+        pin();
+        issue(vissue, sissue);
+        pin();
+        fold(k, sfold, vfold);
+    };
     for (int k = 0;;) {
-        VA = load_v(k + 2);
-        pin();
-        issue(VB, SB);
-        pin();
-        fold(k, SA);
+        step(k, V2, V1, S1, S0, V0);
         if (++k >= nmax) break;
-        VB = load_v(k + 2);
-        pin();
-        issue(VA, SA);
-        pin();
-        fold(k, SB);
+        step(k, V0, V2, S0, S1, V1);
+        if (++k >= nmax) break;
+        step(k, V1, V0, S1, S0, V2);
+        if (++k >= nmax) break;
+        step(k, V2, V1, S0, S1, V0);
+        if (++k >= nmax) break;
+        step(k, V0, V2, S1, S0, V1);
+        if (++k >= nmax) break;
+        step(k, V1, V0, S0, S1, V2);
         if (++k >= nmax) break;
     }
     return total;
