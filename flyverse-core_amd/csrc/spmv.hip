@@ -220,7 +220,16 @@ __global__ __launch_bounds__(1024) void k_spmv_lds(const int* __restrict__ tile_
     if (t < r1 - r0) {
         const int row = r0 + t;
         float total = 0.0f;
-        for (int c = chunk_ptr[row] - c0; c < chunk_ptr[row + 1] - c0; ++c) total = total + xs[c];
+        const int ca = chunk_ptr[row] - c0, cb = chunk_ptr[row + 1] - c0;
+        int c = ca;
+        for (; c + 8 <= cb; c += 8) {              // eight reads in flight, then their adds in order
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = xs[c + q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) total = total + v[q];
+        }
+        for (; c < cb; ++c) total = total + xs[c];
         if (scale != 1.0f) total = total * scale;
         float* yp = y + (int64_t)b * rows + row;
         if (accumulate) total = *yp + total;
