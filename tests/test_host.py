@@ -236,3 +236,51 @@ def test_budget_chunk_follows_measured_costs():
     assert budget_chunk({1: 0.01}, cands, 1e9) == 100             # the whole frame is the largest candidate
     r = StepResult(50.0, 25.0, 500)
     assert r.time_dilation == 2.0 and StepResult(1.0, 0.0, 10).time_dilation == 0.0
+
+
+@pytest.mark.parametrize("norm,out_norm", [("l1", "l1"), ("l2", "l2"), ("l2", "l1")])
+def test_optic_partition_normalisation_and_pair_gains(norm, out_norm):
+    """optic_matrices against a dense float64 restatement of flyverse/optic.py:229-273: the three cell classes, each
+    row divided by max(its L1 or L2 input total, 1), the pair gains applied in rule order to the rate->rate and
+    rate->spiking blocks (anchored regular expressions, double factors), then float32."""
+    from flyhip.optic import OpticParams, optic_matrices
+    conn, retina, layout = synth.fly_graph(2, 0.01)
+    rules = [("OL5", "OL", 1.5), ("OL", "OL59", 0.7), ("CB", "OL", 1.25), ("OL", "CB", 0.5), ("OL1", "OL1", 3.0)]
+    op = OpticParams(tau_by_type=conn.rate_tau, baseline_by_type=conn.rate_baseline, pair_gain=rules, norm=norm,
+                     out_norm=out_norm)
+    m = optic_matrices(conn, retina, op)
+    nrn = conn.neurons
+    types = nrn.type.fillna("").to_numpy()
+    is_pr_type = nrn.type.isin(["R1-R6", "R7p", "R7y", "R7d", "R7_unclear", "R8p", "R8y", "R8d", "R8_unclear",
+                                "R7R8_unclear"]).to_numpy()
+    rate = [i for i in range(conn.n) if nrn.superclass[i] == "ol_intrinsic" and not is_pr_type[i]]
+    placed = set(retina.pr_index.tolist())
+    spk = [i for i in range(conn.n) if not is_pr_type[i] and i not in rate and i not in placed]
+    assert m["rate_idx"].tolist() == rate and m["spk_idx"].tolist() == spk
+    D = conn.W.toarray().astype(np.float64)
+    l1 = D / np.maximum(nrn.in_syn.to_numpy(), 1.0)[:, None]
+    l2 = D / np.maximum(nrn.in_syn_l2.to_numpy(), 1.0)[:, None]
+    into_rate = l2 if norm == "l2" else l1
+
+    def gained(M, post_t, pre_t):
+        M = M.copy()
+        for pre_re, post_re, f in rules:
+            rows = np.array([re.match(post_re, t) is not None for t in post_t])
+            cols = np.array([re.match(pre_re, t) is not None for t in pre_t])
+            sel = np.outer(rows, cols) & (M != 0)
+            M[sel] = M[sel] * f
+        return M.astype(np.float32)
+
+    rt, st = types[rate], types[spk]
+    want = {"M_rr": gained(into_rate[np.ix_(rate, rate)], rt, rt),
+            "M_rs": into_rate[np.ix_(rate, spk)].astype(np.float32),
+            "M_sr": gained((l2 if out_norm == "l2" else l1)[np.ix_(spk, rate)], st, rt),
+            "M_rp": into_rate[np.ix_(rate, retina.pr_index)].astype(np.float32)}
+    assert not np.array_equal(want["M_rr"], into_rate[np.ix_(rate, rate)].astype(np.float32))   # the rules apply
+    assert not np.array_equal(want["M_sr"], (l2 if out_norm == "l2" else l1)[np.ix_(spk, rate)].astype(np.float32))
+    for name, W in want.items():
+        got = m[name]
+        assert np.array_equal(got.toarray(), W), name
+        assert got.has_canonical_format and got.dtype == np.float32
+    assert np.array_equal(m["a"], np.exp(-op.dt_ms / np.array([conn.rate_tau.get(t, op.tau_ms) for t in rt],
+                                                               np.float32)).astype(np.float32))
