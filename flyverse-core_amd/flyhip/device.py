@@ -24,7 +24,9 @@ def default_device() -> torch.device:
 
 
 def resolve(device=None) -> torch.device:
-    return torch.device(device) if device is not None else default_device()
+    if device is None:
+        return default_device()
+    return torch.device(device)
 
 
 def sparse_matrix(D: sp.spmatrix, device, dtype=torch.float32):

@@ -44,13 +44,11 @@ def test_library_exports_every_header_symbol():
 
 def test_reference_signatures_are_registered():
     """The six reference entry points keep flyverse/cuda.py:73-75's argument lists."""
-    P, I, F, S = ct.POINTER(ct.c_void_p), ct.c_int, ct.c_float, ct.c_void_p
-    assert hip.SIGNATURES["launch_lif"] == [P, I, I, I, S]
-    assert hip.SIGNATURES["launch_optic"] == [P, I, I, F, F, F, S]
-    assert hip.SIGNATURES["launch_csr"] == [P, I, I, I, I, S]
-    assert hip.SIGNATURES["launch_events"] == [P, I, I, I, I, S]
-    assert hip.SIGNATURES["launch_means"] == [P, I, I, I, I, S]
-    assert hip.SIGNATURES["launch_optic_dr"] == [P, I, I, S]
+    ptrs, i32, f32, stream = ct.POINTER(ct.c_void_p), ct.c_int, ct.c_float, ct.c_void_p
+    expected = {"launch_lif": [ptrs, i32, i32, i32, stream], "launch_optic": [ptrs, i32, i32, f32, f32, f32, stream],
+                "launch_csr": [ptrs, i32, i32, i32, i32, stream], "launch_events": [ptrs, i32, i32, i32, i32, stream],
+                "launch_means": [ptrs, i32, i32, i32, i32, stream], "launch_optic_dr": [ptrs, i32, i32, stream]}
+    assert {name: hip.SIGNATURES[name] for name in expected} == expected
 
 
 def small_connectome(rng, n=40, density=0.2):
@@ -284,3 +282,19 @@ def test_optic_partition_normalisation_and_pair_gains(norm, out_norm):
         assert got.has_canonical_format and got.dtype == np.float32
     assert np.array_equal(m["a"], np.exp(-op.dt_ms / np.array([conn.rate_tau.get(t, op.tau_ms) for t in rt],
                                                                np.float32)).astype(np.float32))
+
+
+def test_frame_clock_takes_whole_steps_and_carries_the_rest():
+    """floor((carry + frame + 1e-9) / dt) steps and a nonnegative carry (flyverse/fly.py:421-425): 1 ms of 0.1 ms
+    steps is 10 steps (floor division of floats would give 9), 0.25 ms frames of 0.1 ms steps alternate 2, 3."""
+    from flyhip.clock import split_frame, whole_steps
+    assert whole_steps(1.0, 0.1) == 10 and 1.0 // 0.1 == 9.0
+    carry, taken = 0.0, []
+    for _ in range(4):
+        n, carry = split_frame(carry, 0.25, 0.1)
+        taken.append(n)
+        assert carry >= 0.0
+    assert taken == [2, 3, 2, 3]
+    assert split_frame(0.0, 0.0, 0.5) == (0, 0.0)
+    n, carry = split_frame(0.0, 0.3, 0.1)            # 0.1 + 0.1 + 0.1 rounds just under 0.3: the allowance keeps 3
+    assert n == 3 and carry == 0.0
