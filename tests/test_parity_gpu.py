@@ -440,6 +440,48 @@ def test_fly_state_dict_round_trip(graphs):
     for k in ("v", "adapt", "delta_rate"):
         assert torch.equal(getattr(a.optic, k), getattr(b.optic, k)), k
     assert a.brain.spike_counts.sum().item() > 0
+    assert a.t == b.t == a.brain.t
+    other = dict(state, body_ids=state["body_ids"] + 1)
+    with pytest.raises(ValueError, match="connectome does not match"):
+        b.load_state_dict(other)
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_fly_reset_starts_over(graphs):
+    """reset() (flyverse/fly.py:682-706) returns a fly to its built state: clocks at zero, no held radiance, no
+    captured graphs, no Poisson drive; given the same inputs again it gives the bits of a fresh FlyBrain.  reset(rows)
+    rests those flies only."""
+    conn, retina, layout, lp, op = fly_setup(0.02, seed=2)
+    idx, hz = synth.fly_inputs(6, layout, 40)
+
+    def make():
+        return FlyBrain(conn, retina, batch=2, device=DEV, seed=4, lif_params=lp, optic_params=op, cuda_graphs=graphs)
+
+    def run(fb):
+        fb.set_poisson(idx, hz)
+        fb.random_movie(23)
+        for _ in range(3):
+            fb.step(10.0)
+
+    a = make()
+    run(a)
+    a.step(2.5)                                      # leaves a fractional carry behind
+    a.reset()
+    assert a.t == 0.0 and a.brain.step_count == 0 and a.brain.buf_pos == 0 and a._pending_ms == 0.0
+    assert a._radiance is None and not a._graphs and not a.brain._poisson_on
+    assert not bool(a.brain.poisson_p.any())
+    run(a)
+    b = make()
+    run(b)
+    for k in ("v", "g", "rate", "refrac", "spikes", "spike_counts", "drive", "spike_buf"):
+        assert torch.equal(getattr(a.brain, k), getattr(b.brain, k)), k
+    for k in ("v", "adapt", "I_lp", "delta_rate"):
+        assert torch.equal(getattr(a.optic, k), getattr(b.optic, k)), k
+    assert a.t == b.t and a.brain.spike_counts.sum().item() > 0
+    a.reset([1])
+    assert torch.equal(a.brain.v[0], b.brain.v[0]) and bool((a.brain.v[1] == lp.v_rest).all())
+    assert not bool(a.brain.spike_buf[:, 1].any()) and not bool(a.brain.poisson_p[1].any())
+    assert a.t == b.t
 
 
 @pytest.mark.parametrize("graphs", [False, True])
