@@ -6,6 +6,7 @@ defined chunked order for CSR products); CSR products additionally within 1e-5 r
 import math
 
 import numpy as np
+import pandas as pd
 import pytest
 import scipy.sparse as sp
 import torch
@@ -13,6 +14,7 @@ import torch
 import oracle
 from flyhip import hip, synth
 from flyhip.brain import Brain, LIFParams
+from flyhip.connectome import Connectome
 from flyhip.fly import FlyBrain
 from helpers import fly_setup, grouped_retina, oracle_fly, random_csr
 
@@ -171,6 +173,38 @@ def test_pull_half_weights():
     g = T(g0)
     hip.pull(T(W.indptr.astype(np.int32)), T(W.indices.astype(np.int32)), T(wh), T(x), g, 0.9)
     assert_bits(N_(g), ref, "pull half")
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float16"])
+def test_brain_synaptic_input_accumulates_in_float32(dtype):
+    """Brain._add_synaptic_input (flyverse/brain.py:668-692): g += W x with no decay.  Half weights beyond the half
+    range (6 x 20000 = 120000 > 65504) still sum exactly in float32, as flyverse/tests/test_control.py:200-207 checks;
+    on a random graph the result matches the oracle's ascending-presynaptic fold bit for bit."""
+    n = 6
+    c = Connectome(pd.DataFrame({"bodyId": np.arange(n), "type": [f"T{i}" for i in range(n)]}),
+                   sp.csr_matrix(np.ones((n, n), np.float32)))
+    b = Brain(c, LIFParams(weight_dtype=dtype, input_norm_alpha=0, conn_cap=0), device=DEV)
+    b.set_weights(sp.csr_matrix(np.full((n, n), 20000, dtype=np.float32)))
+    b._add_synaptic_input(torch.ones_like(b.g))
+    assert b.g.dtype == torch.float32
+    assert torch.equal(b.g, torch.full_like(b.g, 120000.0))
+    rng = np.random.default_rng(8)
+    n, B = 500, 3
+    W = random_csr(rng, n, n, 0.05)
+    if dtype == "float16":
+        W.data = W.data.astype(np.float16).astype(np.float32)
+    c = Connectome(pd.DataFrame({"bodyId": np.arange(n), "type": ["T"] * n}), sp.csr_matrix((n, n), dtype=np.float32))
+    b = Brain(c, LIFParams(weight_dtype=dtype), device=DEV, batch=B)
+    b.set_weights(W)
+    g0 = rng.normal(0, 1, (B, n)).astype(np.float32)
+    x = (rng.random((B, n)) < 0.3).astype(np.float32)
+    b.g.copy_(T(g0))
+    b._add_synaptic_input(T(x))
+    ref = g0.copy()
+    oracle.events(_csc(W).indptr, _csc(W).indices, _csc(W).data, x, ref)
+    assert_bits(N_(b.g), ref, f"synaptic input {dtype}")
+    with pytest.raises(ValueError):
+        b._add_synaptic_input(T(x[:, :-1]))
 
 
 @pytest.mark.parametrize("B,kernel,range_cols", [(1, "grouped", 0), (3, "grouped", 0), (8, "grouped", 0),
@@ -482,6 +516,32 @@ def test_fly_reset_starts_over(graphs):
     assert torch.equal(a.brain.v[0], b.brain.v[0]) and bool((a.brain.v[1] == lp.v_rest).all())
     assert not bool(a.brain.spike_buf[:, 1].any()) and not bool(a.brain.poisson_p[1].any())
     assert a.t == b.t
+
+
+def test_fly_graphs_follow_weight_replacement():
+    """set_weights invalidates the captured frames (flyverse/fly.py:513-515, test_cuda.py:163-167): after new
+    weights the graph path captures afresh and keeps the bits of the eager path."""
+    conn, retina, layout, lp, op = fly_setup(0.02, seed=2)
+    idx, hz = synth.fly_inputs(6, layout, 40)
+    flies = []
+    for graphs in (False, True):
+        fb = FlyBrain(conn, retina, batch=2, device=DEV, seed=4, lif_params=lp, optic_params=op, cuda_graphs=graphs)
+        fb.set_poisson(idx, hz)
+        fb.random_movie(23)
+        fb.step(10.0)
+        fb.step(10.0)
+        flies.append(fb)
+    eager, graphed = flies
+    old = list(graphed._graphs.values())
+    assert old
+    for fb in flies:
+        fb.brain.set_weights(fb.brain._W_cpu * np.float32(1.5))
+        fb.step(10.0)
+        fb.step(10.0)
+    assert all(entry not in old for entry in graphed._graphs.values())
+    for k in ("v", "g", "rate", "spikes", "spike_counts", "drive"):
+        assert torch.equal(getattr(eager.brain, k), getattr(graphed.brain, k)), k
+    assert eager.brain.spike_counts.sum().item() > 0
 
 
 @pytest.mark.parametrize("graphs", [False, True])
