@@ -86,12 +86,9 @@ def lif_block_bytes(bb, K, B):
 def time_lif_launches(bb):
     """The k_batch_lif launches of one frame (every offset, as the captured frame issues them), each timed with HIP
     events on its launch stream.  Returns (mean duration in ms, mean steps per launch)."""
-    import ctypes as ct
     import torch
     from flyhip.batch import _lib
-    lif = [bb.lif_ptr, bb.lif_pre_off, bb.lif_w, bb.lif_sched, bb.neuron_id, bb.row_flags, bb.v, bb.g, bb.refrac,
-           bb.rate, bb.counts, bb.drive, bb.pois, bb.ring, bb.seed, bb.fly_step, bb.budget, bb.engine_step, bb.P]
-    arr = (ct.c_void_p * len(lif))(*(t.data_ptr() for t in lif))
+    arr = bb.lif_pointers()
     stream = torch.cuda.current_stream()
     fn = _lib(bb.device).launch_batch_lif
     times, ks = [], []

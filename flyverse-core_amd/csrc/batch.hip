@@ -274,6 +274,8 @@ struct BatchLif {
     const uint32_t* budget;  // (64) steps this lane's fly may take in total
     const uint32_t* engine_step; // (1) engine step at the start of the frame (ring slot clock)
     const float* P;          // (12) v_rest, v_reset, v_th, a_m, dt, t_ref, -, -, -, a_r, rate_gain, a_s
+    const uint8_t* act;      // mode 6: (n_rows) bit k = the mask step k of the launch reads is nonzero
+    const uint64_t* zero;    // mode 6: one zero word, the address of every gather whose mask is known to be zero
     int n_rows, lanes, ring_slots, delay, record, mode, row_major;
 };
 
@@ -281,7 +283,8 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, int K) {
-    constexpr bool L5 = MODE == 5;                  // LDS-staged weights
+    constexpr bool L5 = MODE >= 5;                  // LDS-staged weights
+    constexpr bool ACT = MODE == 6;                 // gathers skipped where the activity bits say the mask is zero
     __shared__ float wstage[MODE == 2 ? 4 : 1][64];
     __shared__ uint64_t cmask[MODE == 4 ? 4 : 1][256];
     __shared__ float cweight[MODE == 4 ? 4 : 1][256];
@@ -370,11 +373,20 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                         c.wslot = c.wslot + 1 == R ? 0 : c.wslot + 1;
                     }
                 };
-                // raw mask words; lanes past the row's end are cleared when folded, so nothing waits on the loads here
+                // raw mask words; lanes past the row's end are cleared when folded, so nothing waits on the loads here.
+                // Mode 6: a lane whose activity bit for step k is clear (its presynaptic mask is zero) reads the one
+                // shared zero word instead, so such lanes cost the gather one cache line together; the loads stay
+                // unconditional, so the in-order wait counts are unchanged.
+                uint32_t act_lo = ~0u, act_hi = ~0u;       // activity bytes of chunks 0-3 and 4-7 (rows of G <= 2)
                 auto gather = [&](const Cursor& c, const uint32_t* o, uint64_t* m) {
                     const char* rb = (const char*)a.ring + (size_t)c.rslot * slot_bytes;
+                    const uint32_t aw = c.grp ? act_hi : act_lo;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) m[u] = *(const uint64_t*)(rb + o[u]);
+                    for (int u = 0; u < 4; ++u) {
+                        const uint64_t* src = (const uint64_t*)(rb + o[u]);
+                        if (ACT && !((aw >> (8 * u + c.k)) & 1u)) src = a.zero;
+                        m[u] = *src;
+                    }
                 };
                 float acc = 0.0f;
                 auto fold = [&](const Cursor& c, const uint64_t* m, const float* wv) {
@@ -473,6 +485,16 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                         const int e = min(e0 + c * 64 + lane, e1 - 1);
                         oc[c] = a.pre_off[e];
                         wc[c] = a.w[e];
+                    }
+                    if (ACT) {
+                        // slot-major ring: the byte offset of a mask within a slot is 8 x its row
+                        act_lo = 0u;
+                        act_hi = 0u;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            act_lo |= (uint32_t)a.act[oc[c] >> 3] << (8 * c);
+                            act_hi |= (uint32_t)a.act[oc[4 + c] >> 3] << (8 * c);
+                        }
                     }
                     if (L5) {
                         // the row's weights go to LDS once per launch (after this wavefront's reads of the previous
@@ -809,6 +831,24 @@ __global__ __launch_bounds__(256) void k_reset_lanes(float* v, float* g, float* 
 
 inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
 
+// The activity bits of one k_batch_lif<6> launch (slot-major ring): act[r] bit k is set when the mask of row r in
+// the slot that step k of the launch reads, (T0 + k - delay) mod ring_slots, has any fly's bit set.  Those slots
+// were written by earlier launches (K <= delay), so the bits describe exactly what the launch will read.
+__global__ __launch_bounds__(256) void k_batch_act(const uint64_t* __restrict__ ring, const uint32_t* engine_step,
+                                                   int step_offset, int K, int n_rows, int R, int D,
+                                                   uint8_t* __restrict__ act) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n_rows) return;
+    const uint32_t T0 = *engine_step + (uint32_t)step_offset;
+    uint32_t slot = (T0 + (uint32_t)R - (uint32_t)D) % (uint32_t)R;
+    uint32_t bits = 0u;
+    for (int k = 0; k < K; ++k) {
+        if (ring[(size_t)slot * n_rows + r] != 0ull) bits |= 1u << k;
+        slot = slot + 1 == (uint32_t)R ? 0u : slot + 1;
+    }
+    act[r] = (uint8_t)bits;
+}
+
 }  // namespace
 
 // Up to `delay` LIF steps of every row for 64 flies.  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v, g,
@@ -816,9 +856,12 @@ inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
 // dims = [n_rows, lanes, ring_slots, delay, record, grid, mode, row_major]; mode 0 reads the spike masks through the
 // scalar cache, mode 1 gathers them with vector loads, mode 2 gathers and transposes them, mode 3 transposes with
 // the gathers of the next work item in flight, mode 4 also drops the edges no fly transmits before transposing,
-// mode 5 is mode 3 with the weights staged in LDS behind a -0.0 sentinel (adds without selects)
+// mode 5 is mode 3 with the weights staged in LDS behind a -0.0 sentinel (adds without selects), mode 6 is mode 5
+// with the mask gathers of presynaptic cells that fired in no fly skipped: it needs a slot-major ring, K <= 8 and
+// p[19] = an activity buffer of (n_rows rounded up to a multiple of 8) + 8 bytes whose last 8 bytes are zero (the
+// bits are written by a k_batch_act launch just before the LIF launch)
 // (fold_edges, fold_edges_gather, fold_edges_transposed, fold_chunk_permute: the same additions in the same order,
-// so the same result).
+// so the same result; a skipped gather is of a zero mask, which adds nothing).
 FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream) {
     BatchLif a;
     a.ptr = (const int*)p[0]; a.pre_off = (const uint32_t*)p[1]; a.w = (const float*)p[2]; a.sched = (const int*)p[3];
@@ -832,16 +875,30 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     a.mode = dims[6];
     a.row_major = dims[7];
     if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
-        grid <= 0 || a.mode < 0 || a.mode > 5)
+        grid <= 0 || a.mode < 0 || a.mode > 6)
         return FH_ERR_ARGS;
+    a.act = nullptr;
+    a.zero = nullptr;
+    if (a.mode == 6) {
+        if (a.row_major || K > 8 || p[19] == nullptr) return FH_ERR_ARGS;
+        a.act = (const uint8_t*)p[19];
+        a.zero = (const uint64_t*)(a.act + (((size_t)a.n_rows + 7) & ~(size_t)7));
+    }
     if (a.n_rows == 0 || K == 0) return 0;
+    if (a.mode == 6) {
+        hipLaunchKernelGGL(k_batch_act, dim3(nblk(a.n_rows)), dim3(256), 0, (hipStream_t)stream, a.ring,
+                           a.engine_step, step_offset, K, a.n_rows, a.ring_slots, a.delay, (uint8_t*)p[19]);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
+    }
     switch (a.mode) {
         case 0: hipLaunchKernelGGL(k_batch_lif<0>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 1: hipLaunchKernelGGL(k_batch_lif<1>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 2: hipLaunchKernelGGL(k_batch_lif<2>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 3: hipLaunchKernelGGL(k_batch_lif<3>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 4: hipLaunchKernelGGL(k_batch_lif<4>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
-        default: hipLaunchKernelGGL(k_batch_lif<5>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
+        case 5: hipLaunchKernelGGL(k_batch_lif<5>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
+        default: hipLaunchKernelGGL(k_batch_lif<6>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
     }
     FH_LAUNCH_CHECK();
 }

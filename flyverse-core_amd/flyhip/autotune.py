@@ -38,7 +38,7 @@ def batch_key(bb) -> str:
     return f"batch:{dev.name}:{dev.multi_processor_count}:rows{bb.n_rows}:nnz{bb.W_lif.nnz}:rate{bb.n_rate}:D{bb.n_delay}"
 
 
-def tune_batch(bb, frames: int = 5, fold_modes=(5,), steps=(4, 5, 6), grids=(32, 64), use_cache=True,
+def tune_batch(bb, frames: int = 5, fold_modes=(5, 6), steps=(4, 5, 6), grids=(32, 64), use_cache=True,
                log=None, warm: int = 30) -> dict:
     """Time each (fold mode, steps per launch, grid per CU) on `bb` (a BatchBrain with rollouts queued) and leave
     it configured with the fastest.  `warm` frames run first: flies start at rest and their firing (the work per
@@ -55,7 +55,7 @@ def tune_batch(bb, frames: int = 5, fold_modes=(5,), steps=(4, 5, 6), grids=(32,
     if warm:
         bb.run(warm)
     for mode, K, gpc in itertools.product(fold_modes, steps, grids):
-        if K > bb.n_delay:
+        if K > bb.n_delay or (mode == 6 and (bb.ring_layout != "slot" or K > 8)):
             continue
         bb.configure(fold_mode=mode, steps_per_launch=K, grid_per_cu=gpc)
         bb.run(1)                                   # capture + first replay

@@ -119,8 +119,11 @@ int launch_pull_bits(void** p, int n, int n_full, int batch, float a_s, int flag
  * with per-fly step counters, so rollouts of uneven length share one captured graph. */
 
 /* K LIF steps (global steps *engine_step + step_offset ...).  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v,
- * g, refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12)];
- * dims = [n_rows, lanes, ring_slots, delay, record, grid, fold mode 0|1|2, row_major ring]. */
+ * g, refrac, rate, counts, drive, pois, ring, seed, fly_step, budget, engine_step, P(12), act];
+ * dims = [n_rows, lanes, ring_slots, delay, record, grid, fold mode 0-6, row_major ring].  Every fold mode performs
+ * the same additions in the same order.  Mode 6 (slot-major ring, K <= 8) first writes one activity byte per row into
+ * act (bit k: the mask step k reads is nonzero) and then skips the mask gathers of the presynaptic cells that fired
+ * in no fly; act holds (n_rows rounded up to a multiple of 8) + 8 bytes, the last 8 zero.  Other modes ignore p[19]. */
 int launch_batch_lif(void** p, const int* dims, int step_offset, int K, void* stream);
 
 /* y (rows, 64) = scale * W x (cols, 64), chunked order of launch_csr.  p = [ptr, idx, w, sched, x, y].

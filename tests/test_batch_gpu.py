@@ -32,7 +32,7 @@ def rollouts(layout, n, frames_lo, frames_hi, frame_steps, seed=0):
 @pytest.mark.parametrize("graph,mode,grouped", [(False, 0, False), (True, 1, False), (True, 2, False), (False, 2, False),
                                                 (True, 3, False), (False, 3, False), (True, 4, False),
                                                 (False, 4, False), (True, 5, False), (False, 5, False),
-                                                (True, 5, True)])
+                                                (True, 5, True), (True, 6, False), (False, 6, True)])
 def test_lanes_equal_solo_oracle_runs_with_refill(graph, mode, grouped):
     conn, retina, layout, lp, op = fly_setup(0.02, seed=2)
     if grouped:
@@ -80,7 +80,7 @@ def test_full_scale_batch_is_deterministic_and_matches_plain_path():
     op = OpticParams(tau_by_type=conn.rate_tau, baseline_by_type=conn.rate_baseline, pair_gain=[])
     rs = rollouts(layout, 64, 3, 3, 100, seed=5)
     states = []
-    for rep, mode in enumerate((2, 4, 5)):
+    for rep, mode in enumerate((2, 4, 5, 6)):
         bb = BatchBrain(conn, retina, lp, op, device=DEV, fold_mode=mode)
         bb.submit(rs)
         bb.run(3)
