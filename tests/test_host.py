@@ -298,3 +298,29 @@ def test_frame_clock_takes_whole_steps_and_carries_the_rest():
     assert split_frame(0.0, 0.0, 0.5) == (0, 0.0)
     n, carry = split_frame(0.0, 0.3, 0.1)            # 0.1 + 0.1 + 0.1 rounds just under 0.3: the allowance keeps 3
     assert n == 3 and carry == 0.0
+
+
+def test_select_criteria_and_dictionary_selections():
+    """Connectome.select (flyverse/connectome.py:604-616): scalars by equality, collections by membership, '~'
+    strings as regular expressions searched anywhere with missing values as empty; all criteria must hold.  A
+    dictionary passed to indices() or subset() selects the same rows."""
+    neurons = pd.DataFrame({"bodyId": [10, 11, 12, 13, 14, 15],
+                            "type": ["DNa01", "DNa02", "DNa03", None, "xDNa01", "LC4"],
+                            "somaSide": ["L", "R", "L", "L", "R", "L"],
+                            "superclass": ["descending", "descending", "descending", "ol_intrinsic", "descending",
+                                           "visual_projection"]})
+    c = Connectome(neurons, sp.csr_matrix((6, 6), dtype=np.float32))
+    assert c.select(somaSide="L").tolist() == [0, 2, 3, 5]
+    assert c.select(type="~^DNa0[12]").tolist() == [0, 1]          # anchored at the start of the text
+    assert c.select(type="~DNa01").tolist() == [0, 4]              # found anywhere, as str.contains does
+    assert c.select(type="~^$").tolist() == [3]                    # the missing type reads as ""
+    assert c.select(type=["LC4", "DNa03"]).tolist() == [2, 5]
+    assert c.select(type={"LC4"}, somaSide="L").tolist() == [5]
+    assert c.select(type="~^DNa0[12]", somaSide="L").tolist() == [0]
+    assert c.select(superclass="nothing").tolist() == []
+    assert c.select().tolist() == list(range(6))
+    assert c.indices({"somaSide": "R"}).tolist() == [1, 4]
+    sub = c.subset({"superclass": "descending", "somaSide": "L"})
+    assert sub.neurons.bodyId.tolist() == [10, 12]
+    with pytest.raises(KeyError):
+        c.select(no_such_column=1)
