@@ -324,3 +324,17 @@ def test_select_criteria_and_dictionary_selections():
     assert sub.neurons.bodyId.tolist() == [10, 12]
     with pytest.raises(KeyError):
         c.select(no_such_column=1)
+
+
+def test_batch_launch_shapes():
+    """The launch shapes the batched engine accepts: modes 0-6, 1..n_delay steps, mode 6 only on the slot-major
+    ring with at most 8 steps (its activity bits are one byte per row)."""
+    from flyhip.batch import check_launch_shape
+    for mode in range(7):
+        check_launch_shape(mode, 6, 18, "slot")
+    check_launch_shape(5, 18, 18, "row")
+    check_launch_shape(6, 8, 18, "slot")
+    for bad in [(7, 6, 18, "slot"), (-1, 6, 18, "slot"), (5, 0, 18, "slot"), (5, 19, 18, "slot"),
+                (6, 9, 18, "slot"), (6, 6, 18, "row")]:
+        with pytest.raises(ValueError):
+            check_launch_shape(*bad)
