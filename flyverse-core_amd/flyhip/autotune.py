@@ -38,12 +38,14 @@ def batch_key(bb) -> str:
     return f"batch:{dev.name}:{dev.multi_processor_count}:rows{bb.n_rows}:nnz{bb.W_lif.nnz}:rate{bb.n_rate}:D{bb.n_delay}"
 
 
-def tune_batch(bb, frames: int = 5, fold_modes=(5, 6), steps=(4, 5, 6), grids=(32, 64), use_cache=True,
+def tune_batch(bb, frames: int = 5, fold_modes=None, steps=(4, 5, 6), grids=(64, 96, 128), use_cache=True,
                log=None, warm: int = 30) -> dict:
     """Time each (fold mode, steps per launch, grid per CU) on `bb` (a BatchBrain with rollouts queued) and leave
     it configured with the fastest.  `warm` frames run first: flies start at rest and their firing (the work per
     step) rises over the first tens of frames, so trials compare at a settled activity.  Returns {"config": ...,
     "ms_per_frame": ..., "trials": [...]}."""
+    if fold_modes is None:               # mode 6 needs the slot-major ring; mode 5 is the fastest without it
+        fold_modes = (6,) if bb.ring_layout == "slot" else (5,)
     key = batch_key(bb)
     table = _load() if use_cache else {}
     if key in table:
