@@ -11,11 +11,23 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 import torch
+from torch.utils.checkpoint import checkpoint as _checkpoint
+
+
+def _step(v, adapt, w, a, b, P_t, rows, cols, g_rr, g_ad, a_ad):
+    """One step of the rate dynamics (flyverse/optic.py:528-551): returns (v', adapt')."""
+    dr = (v + b).clamp(0.0, 1.0) - b
+    y = torch.zeros_like(v).index_add(0, rows, w[:, None] * dr[cols])
+    inp = g_rr * y + P_t - g_ad * adapt
+    return inp + (v - inp) * a, dr + (adapt - dr) * a_ad
 
 
 def loss64(tau, bias, gain, W0: sp.csr_matrix, unit_type, P, target, readout, dt=5.0, g_rr=1.0, g_ad=1.0,
-           adapt_tau=400.0):
-    """tau, bias (K,), gain (K, K) float64 tensors (may require grad); P (T, n, B); target (T, R, B)."""
+           adapt_tau=400.0, per_step_checkpoint=False):
+    """tau, bias (K,), gain (K, K) float64 tensors (may require grad); P (T, n, B); target (T, R, B).
+
+    per_step_checkpoint: keep only (v, adapt) per step for the backward and recompute each step's product there
+    (torch.utils.checkpoint), so the full-size problem fits in host memory; the arithmetic is the same."""
     W0 = sp.coo_matrix(W0)
     rows = torch.as_tensor(W0.row.astype(np.int64))
     cols = torch.as_tensor(W0.col.astype(np.int64))
@@ -26,19 +38,19 @@ def loss64(tau, bias, gain, W0: sp.csr_matrix, unit_type, P, target, readout, dt
     a = torch.exp(-dt / tau)[ut][:, None]
     b = bias[ut][:, None]
     a_ad = float(np.exp(-dt / adapt_tau))
-    P = torch.as_tensor(P, dtype=torch.float64)
-    target = torch.as_tensor(target, dtype=torch.float64)
     ro = torch.as_tensor(np.asarray(readout, np.int64))
     T, n, B = P.shape
     v = torch.zeros(n, B, dtype=torch.float64)
     adapt = torch.zeros(n, B, dtype=torch.float64)
     total = torch.zeros((), dtype=torch.float64)
     for t in range(T):
-        dr = (v + b).clamp(0.0, 1.0) - b
-        y = torch.zeros(n, B, dtype=torch.float64).index_add(0, rows, w[:, None] * dr[cols])
-        inp = g_rr * y + P[t] - g_ad * adapt
-        v = inp + (v - inp) * a
-        adapt = dr + (adapt - dr) * a_ad
+        P_t = torch.as_tensor(np.asarray(P[t]), dtype=torch.float64)
+        args = (v, adapt, w, a, b, P_t, rows, cols, g_rr, g_ad, a_ad)
+        if per_step_checkpoint and torch.is_grad_enabled():
+            v, adapt = _checkpoint(_step, *args, use_reentrant=False)
+        else:
+            v, adapt = _step(*args)
         r = (v + b).clamp(0.0, 1.0)
-        total = total + ((r[ro] - target[t]) ** 2).sum()
+        tgt = torch.as_tensor(np.asarray(target[t]), dtype=torch.float64)
+        total = total + ((r[ro] - tgt) ** 2).sum()
     return total / (T * B * len(readout))

@@ -63,3 +63,21 @@ def grouped_retina(retina, per_column=3, seed=0):
     n = len(retina.pr_index)
     col = rng.permutation(n) // per_column
     return Retina(retina.pr_index, retina.pr_sens, col, rng.choice(5, n, p=[.6, .1, .1, .1, .1]), int(col.max()) + 3)
+
+
+C4_TARGET_SEED, C4_READOUT_SEED = 9, 8
+
+
+def c4_problem(T=None):
+    """The full-size differentiable optic-lobe problem (89,390 units, ~6 M edges, 65 types, 32 grating movies,
+    734 readout units, dt 5 ms): (W0, unit_type, P, target, readout, tau, bias, gain).  The network, movies and
+    initial parameters are flyhip.optic_data's; the targets are seeded uniforms on [0, 0.3] (the readout rates' range)
+    so that the problem is rebuilt identically without a GPU.  T: first T steps only (default all 200)."""
+    from flyhip import optic_data as od
+    T = od.T_STEPS if T is None else T
+    W0, ut, W_rp, pos = od.optic_network()
+    P = od.photoreceptor_drive(od.grating_intensity(pos, T), W_rp)
+    readout = np.sort(np.random.default_rng(C4_READOUT_SEED).choice(W0.shape[0], od.N_READOUT, replace=False))
+    target = np.random.default_rng(C4_TARGET_SEED).uniform(0.0, 0.3, (T, od.N_READOUT, od.MOVIES)).astype(np.float32)
+    tau, bias, gain = od.initial_params()
+    return W0, ut, P, target, readout, tau, bias, gain
