@@ -149,3 +149,86 @@ def test_adjoint_edge_partials_from_the_transposed_product():
     before = prob.lv.clone()
     prob._back_rows(0, 1, f_back, None)                                 # no eg: the step's other outputs only
     assert not torch.equal(before, prob.lv)
+
+
+@pytest.mark.gpu
+def test_adjoint_forward_is_the_inference_substep_bit_for_bit():
+    """The training forward evaluates the inference expressions (the invariant of the reference's
+    tests/test_surrogate.py:178-183, grad path == inference path bit for bit): every step's v, adapt and dr of
+    OpticAdjoint's FP32 forward equal the CPU oracle's inference substep (oracle.spmv in the chunked order of the
+    CSR products, then oracle.optic_update, flyverse/optic.py:525-551 with no spiking feedback), given the same
+    per-unit coefficients and shaped weights."""
+    import oracle
+    from flyhip.adjoint import OpticAdjoint
+    n, K, T = 3000, 7, 12
+    W0, ut, P, target, readout, tau, bias, gain = small_problem(n=n, K=K, T=T, nnz=90_000, seed=5)
+    prob = OpticAdjoint(W0, ut, K, P, target, readout, checkpoint=None, device="cuda")
+    t = lambda x: torch.as_tensor(x, device="cuda")
+    prob.loss(t(tau), t(bias), t(gain))
+    torch.cuda.synchronize()
+    a, b = prob.a.cpu().numpy(), prob.b.cpu().numpy()
+    fidx, fw = prob.fidx.cpu().numpy(), prob.fw.cpu().numpy()
+    w = fw[fidx >= 0]                                          # the shaped weights in CSR order
+    W = sp.csr_matrix(W0, dtype=np.float32)
+    W.sum_duplicates()
+    W.sort_indices()
+    assert np.array_equal(fidx[fidx >= 0], W.indices)
+    ptr = W.indptr.astype(np.int64)
+    v = np.zeros((32, n), np.float32)
+    adapt = np.zeros((32, n), np.float32)
+    dr = oracle.optic_dr(v, b)
+    zero = np.zeros((32, n), np.float32)
+    c = prob.c
+    for step in range(T):
+        y = oracle.spmv(ptr, W.indices, w, dr)
+        oracle.optic_update(v, adapt, dr, y, np.ascontiguousarray(P[step].T), zero, a, b, c.gain_rr, c.adapt_gain,
+                            prob.a_ad)
+        for name, got, want in (("v", prob.S_v[step + 1], v), ("adapt", prob.S_a[step + 1], adapt),
+                                ("dr", prob.S_dr[step + 1], dr)):
+            assert np.array_equal(got.cpu().numpy().T, want), (step, name)
+
+
+@pytest.mark.gpu
+def test_full_size_adjoint_matches_the_float64_fixture():
+    """The full-size training problem (89,390 units, 5,995,830 edges, 65 types, 32 movies, 200 steps, checkpoints
+    every 20 steps) against the committed float64 autograd loss and gradient (tests/golden/make_c4_gradient.py):
+    loss within 1e-5 relative, each gradient group within 1e-4 relative; keeping the whole trajectory instead gives
+    the same bits."""
+    from pathlib import Path
+    from flyhip.adjoint import OpticAdjoint
+    from helpers import c4_problem
+    fx = np.load(Path(__file__).parent / "golden" / "c4_gradient.npz")
+    W0, ut, P, target, readout, tau, bias, gain = c4_problem()
+    t = lambda x: torch.as_tensor(x, device="cuda")
+    results = []
+    for ck in (20, None):
+        prob = OpticAdjoint(W0, ut, 65, P, target, readout, checkpoint=ck, device="cuda")
+        L = prob.loss(t(tau), t(bias), t(gain)).clone()
+        g = [x.clone() for x in prob.gradients()]
+        results.append((L, g))
+        del prob
+        torch.cuda.empty_cache()
+    (L, g), (L2, g2) = results
+    assert torch.equal(L, L2)
+    for a_, b_ in zip(g, g2):
+        assert torch.equal(a_, b_)
+    L64 = float(fx["loss"])
+    assert abs(float(L) - L64) <= 1e-5 * abs(L64), (float(L), L64)
+    for name, got, ref in (("tau", g[0], fx["g_tau"]), ("bias", g[1], fx["g_bias"]), ("gain", g[2], fx["g_gain"])):
+        assert rel(got.cpu().numpy(), ref) <= 1e-4, (name, rel(got.cpu().numpy(), ref))
+
+
+def test_c4_fixture_gradient_agrees_with_its_finite_differences():
+    """The committed full-size float64 gradient against the float64 central differences of the same loss that the
+    fixture script took for two entries of each parameter group: within 1e-4 of the entry (floor 1e-6 of the group's
+    largest entry).  Not tighter: a bias shifts units across the kinks of clip(v + b, 0, 1), where a difference
+    quotient averages the slopes on both sides while autograd takes torch's one-sided clamp gradient."""
+    from pathlib import Path
+    fx = np.load(Path(__file__).parent / "golden" / "c4_gradient.npz")
+    groups = (fx["g_tau"].reshape(-1), fx["g_bias"].reshape(-1), fx["g_gain"].reshape(-1))
+    assert fx["g_gain"].shape == (65, 65) and groups[0].size == 65 and groups[1].size == 65
+    assert sorted(set(int(g) for g in fx["fd_group"])) == [0, 1, 2]
+    for grp, k, fd in zip(fx["fd_group"], fx["fd_index"], fx["fd_value"]):
+        g = groups[int(grp)]
+        tol = 1e-4 * max(abs(g[int(k)]), 1e-2 * np.abs(g).max())
+        assert abs(fd - g[int(k)]) <= tol, (int(grp), int(k), fd, g[int(k)])
