@@ -18,13 +18,13 @@
 // Two drivers share the window body: launch_packed_window runs one window per launch (the kernel boundary is the
 // synchronisation; a rollout is a captured graph of such launches with the step base on the device), and
 // launch_packed_run runs a whole rollout in one cooperative launch with a grid barrier between windows.
-#include "common.h"
+#include "chain.h"
 
 #include <type_traits>
 
 namespace {
 
-constexpr int kSlots = 64;
+using fh_chain::kSlots;
 
 struct Packed {
     const int* ptr;          // (n + 1) post-major CSR over all packed cells
@@ -81,108 +81,15 @@ __device__ __forceinline__ void grid_barrier(const Packed& a, unsigned nblocks, 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-constexpr int kMaxWindow = 21;          // 2 * window + delay <= 64 with window <= delay
-constexpr int kShortEdges = 8 * kWave;  // cells with more in-edges are "long"
+using fh_chain::Edges;
+using fh_chain::kMaxWindow;
+using fh_chain::kShortEdges;
+using fh_chain::Window;
 constexpr int kVlongChunks = 36;        // chunks per wavefront in k_packed_vlong (cells of up to 9216 in-edges)
 
-// The window's view of the spike history: bit k of bits() = the cell fired at step T0 - delay + k (k < 32).
-struct Window {
-    int T0, K, s0;
-    float a_s;
-    __device__ Window(const Packed& a, int t0, int k) : T0(t0), K(k), s0((t0 - a.delay) & (kSlots - 1)), a_s(a.P[11]) {}
-    __device__ __forceinline__ uint32_t bits(uint64_t w) const {
-        return (uint32_t)(s0 ? (w >> s0) | (w << (kSlots - s0)) : w);
-    }
-};
-
-// The synaptic chain of one cell over the window, lanes over its in-edges [e0, e1) (at most NC * 64): g_k =
-// g_{k-1} * a_s, then the weights of the in-edges whose presynaptic cell fired at step T0 + k - delay, added in
-// edge order (ascending presynaptic index).  Each edge's weight and history word are read once for the whole window.
-template <int NC>
-__device__ __forceinline__ void load_edges(const Packed& a, int e0, int e1, int lane, int* pre, float* wc) {
-    const int last = max(e1 - 1, e0);
-#pragma unroll
-    for (int u = 0; u < NC; ++u) {
-        const int e = e0 + u * kWave + lane;
-        pre[u] = e1 > e0 ? a.idx[min(e, last)] : 0;
-        wc[u] = e1 > e0 ? a.w[min(e, last)] : 0.0f;
-    }
-}
-
-// the steps from the window's bits of each edge (bit k: the presynaptic cell fired at step T0 - delay + k);
-// returns g_k in lane k
-template <int NC>
-__device__ __forceinline__ float chain_steps_bits(const Window& W, int e0, int e1, float acc, int lane,
-                                                  const float* wc, const uint32_t* hist) {
-    const int nch = (e1 - e0 + kWave - 1) / kWave;
-    float gk = 0.0f;
-    for (int k = 0; k < W.K; ++k) {
-        acc = acc * W.a_s;
-#pragma unroll
-        for (int u = 0; u < NC; ++u) {
-            if (u < nch) {
-                uint64_t m = __builtin_amdgcn_ballot_w64((hist[u] >> k) & 1u);
-                while (m) {
-                    const int j = __builtin_ctzll(m);
-                    acc = acc + fh_readlane(wc[u], j);
-                    m &= m - 1;
-                }
-            }
-        }
-        gk = lane == k ? acc : gk;
-    }
-    return gk;
-}
-
-template <int NC>
-__device__ __forceinline__ float g_chain_cached(const Packed& a, const Window& W, int e0, int e1, float acc, int lane) {
-    int pre[NC];
-    float wc[NC];
-    load_edges<NC>(a, e0, e1, lane, pre, wc);
-    // history words 8 chunks at a time, folded to the window's bits at once (few registers for long cells)
-    uint32_t hist[NC];
-#pragma unroll
-    for (int u0 = 0; u0 < NC; u0 += 8) {
-        uint64_t raw[8];
-#pragma unroll
-        for (int v = 0; v < 8 && u0 + v < NC; ++v) raw[v] = a.ring[pre[u0 + v]];
-#pragma unroll
-        for (int v = 0; v < 8 && u0 + v < NC; ++v)
-            hist[u0 + v] = e0 + (u0 + v) * kWave + lane < e1 ? W.bits(raw[v]) : 0u;
-    }
-    return chain_steps_bits<NC>(W, e0, e1, acc, lane, wc, hist);
-}
-
-// The same chain for a cell of any length, streaming its in-edges (256 at a time) once per step.
-__device__ __forceinline__ float g_chain_stream(const Packed& a, const Window& W, int e0, int e1, float acc, int lane) {
-    float gk = 0.0f;
-    for (int k = 0; k < W.K; ++k) {
-        acc = acc * W.a_s;
-        for (int base = e0; base < e1; base += 4 * kWave) {
-            int pre[4];
-            bool act[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = base + u * kWave + lane;
-                pre[u] = e < e1 ? a.idx[e] : -1;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) act[u] = pre[u] >= 0 && ((W.bits(a.ring[max(pre[u], 0)]) >> k) & 1u);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (base + u * kWave >= e1) break;
-                const float c = act[u] ? a.w[base + u * kWave + lane] : 0.0f;
-                uint64_t m = __builtin_amdgcn_ballot_w64(act[u]);
-                while (m) {
-                    const int j = __builtin_ctzll(m);
-                    acc = acc + fh_readlane(c, j);
-                    m &= m - 1;
-                }
-            }
-        }
-        gk = lane == k ? acc : gk;
-    }
-    return gk;
+__device__ __forceinline__ Edges edges_of(const Packed& a) { return Edges{a.idx, a.w, a.ring}; }
+__device__ __forceinline__ Window window_of(const Packed& a, int t0, int k) {
+    return Window((uint32_t)t0, k, a.delay, a.P[11]);
 }
 
 // One window: steps T0 .. T0 + K - 1 of every cell.  A wavefront takes a block of 64 consecutive cells (blocks in
@@ -202,7 +109,8 @@ __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
     float* const gb = gbuf[threadIdx.x >> 6];
     const float v_rest = a.P[0], v_reset = a.P[1], v_th = a.P[2], am = a.P[3], dt = a.P[4], t_ref = a.P[5];
     const float ar = a.P[9], rg = a.P[10];
-    const Window W(a, T0, K);
+    const Window W = window_of(a, T0, K);
+    const Edges E = edges_of(a);
     for (int bi = wave; bi < a.words; bi += nwaves) {
         const int wd = a.sched[bi];
         const int c0 = wd * 64;
@@ -216,15 +124,15 @@ __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
             const float g0 = fh_readlane(gl, i);
             float gk;
             if (e1 - e0 <= 4 * kWave) {
-                gk = g_chain_cached<4>(a, W, e0, e1, g0, lane);
+                gk = fh_chain::g_chain_cached<4>(E, W, e0, e1, g0, lane);
             } else if (e1 - e0 <= kShortEdges) {
-                gk = g_chain_cached<8>(a, W, e0, e1, g0, lane);
+                gk = fh_chain::g_chain_cached<8>(E, W, e0, e1, g0, lane);
             } else {
                 const int li = LONG_BUF && a.n_long > 0 ? a.long_id[c0 + i] : -1;
                 if (li >= 0)
                     gk = lane < K ? a.long_g[(int64_t)li * kMaxWindow + lane] : 0.0f;
                 else
-                    gk = g_chain_stream(a, W, e0, e1, g0, lane);
+                    gk = fh_chain::g_chain_stream(E, W, e0, e1, g0, lane);
             }
             if (lane < K) gb[i * kMaxWindow + lane] = gk;
         }
@@ -288,18 +196,19 @@ __global__ __launch_bounds__(256) void k_packed_window(Packed a, const int* t_ba
 __global__ __launch_bounds__(256) void k_packed_long(Packed a, const int* t_base, int off, int K) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const Window W(a, *t_base + off, K);
+    const Window W = window_of(a, *t_base + off, K);
+    const Edges E = edges_of(a);
     for (int i = a.n_vlong + wave; i < a.n_long; i += gridDim.x * 4) {
         const int r = a.long_rows[i];
         const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
         const float g0 = a.g[r];
         float gk;
         if (e1 - e0 <= 16 * kWave)
-            gk = g_chain_cached<16>(a, W, e0, e1, g0, lane);
+            gk = fh_chain::g_chain_cached<16>(E, W, e0, e1, g0, lane);
         else if (e1 - e0 <= 32 * kWave)
-            gk = g_chain_cached<32>(a, W, e0, e1, g0, lane);
+            gk = fh_chain::g_chain_cached<32>(E, W, e0, e1, g0, lane);
         else
-            gk = g_chain_stream(a, W, e0, e1, g0, lane);
+            gk = fh_chain::g_chain_stream(E, W, e0, e1, g0, lane);
         if (lane < K) a.long_g[(int64_t)i * kMaxWindow + lane] = gk;
     }
 }
@@ -313,60 +222,11 @@ __global__ __launch_bounds__(256) void k_packed_vlong(Packed a, const int* t_bas
     __shared__ float carry;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const Window W(a, *t_base + off, K);
+    const Window W = window_of(a, *t_base + off, K);
+    const Edges E = edges_of(a);
     for (int i = blockIdx.x; i < a.n_vlong; i += gridDim.x) {
         const int r = a.long_rows[i];
-        const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
-        const int nch = (e1 - e0 + kWave - 1) / kWave;
-        const float g0 = a.g[r];
-        if (nch > 4 * NCW) {
-            if (wv == 0) {
-                const float gk = g_chain_stream(a, W, e0, e1, g0, lane);
-                if (lane < K) a.long_g[(int64_t)i * kMaxWindow + lane] = gk;
-            }
-            __syncthreads();
-            continue;
-        }
-        const int mych = min(max(nch - wv * NCW, 0), NCW);
-        int pre[NCW];
-        float wc[NCW];
-        uint32_t hist[NCW];
-#pragma unroll
-        for (int u = 0; u < NCW; ++u) {
-            const int e = e0 + (wv * NCW + u) * kWave + lane;
-            const bool val = u < mych && e < e1;
-            pre[u] = val ? a.idx[e] : 0;
-            wc[u] = val ? a.w[e] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < NCW; ++u) {
-            const bool val = u < mych && e0 + (wv * NCW + u) * kWave + lane < e1;
-            hist[u] = val ? W.bits(a.ring[pre[u]]) : 0u;
-        }
-        float acc = g0, gk = 0.0f;
-        for (int k = 0; k < K; ++k) {
-            for (int seg = 0; seg < 4; ++seg) {
-                if (wv == seg) {
-                    float x = seg == 0 ? acc * W.a_s : carry;
-#pragma unroll
-                    for (int u = 0; u < NCW; ++u) {
-                        if (u < mych) {
-                            uint64_t m = __builtin_amdgcn_ballot_w64((hist[u] >> k) & 1u);
-                            while (m) {
-                                const int j = __builtin_ctzll(m);
-                                x = x + fh_readlane(wc[u], j);
-                                m &= m - 1;
-                            }
-                        }
-                    }
-                    if (lane == 0) carry = x;
-                }
-                __syncthreads();
-            }
-            acc = carry;
-            gk = lane == k ? acc : gk;
-            __syncthreads();
-        }
+        const float gk = fh_chain::g_chain_workgroup<NCW>(E, W, a.ptr[r], a.ptr[r + 1], a.g[r], lane, wv, &carry);
         if (wv == 0 && lane < K) a.long_g[(int64_t)i * kMaxWindow + lane] = gk;
     }
 }

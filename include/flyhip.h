@@ -112,6 +112,19 @@ int launch_lif_bits(void** p, int n, int batch, int flags, uint64_t seed, uint32
  * workgroups of the persistent launch.  Replaces g.mul_(a_s) + event_scatter, flyverse/brain.py:875-876. */
 int launch_pull_bits(void** p, int n, int n_full, int batch, float a_s, int flags, int grid, void* stream);
 
+/* K <= n_delay LIF steps of the plain (batch, n) brain in one window, from global step *step_base + step (step_base a
+ * device uint32 or null).  Replaces K iterations of Brain._step_cuda (flyverse/brain.py:873-881: g.mul_(a_s),
+ * cuda.event_scatter, torch.rand, cuda.lif_update), with the same additions and updates as K steps of launch_pull +
+ * launch_lif_rng.  Spikes are read from 64-step history words (bit t % 64 of hist[b, c]: cell c of fly b fired at
+ * step t), so each in-edge's weight and presynaptic word is read once per window; the spikes of every step are also
+ * written into the delay ring spike_buf from slot slot0 on.  p = [ptr (n+1), idx, w (float), sched (ceil(n/16)
+ * blocks of 16 cells), v, g, refrac, rate, counts, adapt, spikes, drive, pois, active (n), spike_buf (delay, batch,
+ * n), hist uint64 (batch, n), P (11), long_rows (n_long: every cell with more than 512 in-edges, most first)];
+ * dims = [n, batch, K, delay, slot0, n_long, n_vlong (the first n_vlong long cells have more than 2048 in-edges),
+ * flags (1 Poisson, 4 counts, 8 adaptation)].  Requires 1 <= K <= delay, 2 * delay + K <= 64 and 0 <= slot0 < delay. */
+int launch_lif_window(void** p, const int* dims, uint64_t seed, float a_s, const uint32_t* step_base, uint32_t step,
+                      void* stream);
+
 /* ------------------------------------------------------------------ batched engine: 64 flies, fly-minor (rows, 64)
  * State is (rows, 64): lane b of row r is fly b's copy of cell r.  Spikes are 64-bit masks (one bit per fly) in a
  * ring of ring_slots >= 2 * n_delay entries.  One launch runs K <= n_delay LIF steps of every row.  Together these
