@@ -37,6 +37,45 @@ struct Window {
     }
 };
 
+// (((acc + x_0) + x_1) + ... + x_{n-1}) for the n <= 63 values in lanes 0..n-1 of x, every addition rounded as in
+// a sequential loop, without moving values to scalar registers: a wavefront shift by one lane (DPP wave_shr:1) per
+// round hands lane i's running value to lane i + 1, which adds x_i; lane 0 keeps acc (it adds -0.0, which leaves
+// every value unchanged).  After round r lanes 0..r hold their final values, so after n rounds lane n holds the fold.
+__device__ __forceinline__ float fold_lanes(float acc, float x, int n) {
+    constexpr int kWaveShr1 = 0x138;
+    const float xs = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(-0.0f), __float_as_int(x), kWaveShr1, 0xf, 0xf, false));
+    float y = acc;
+    for (int i = 0; i < n; ++i) {
+        const float t = __int_as_float(
+            __builtin_amdgcn_update_dpp(__float_as_int(y), __float_as_int(y), kWaveShr1, 0xf, 0xf, false));
+        y = t + xs;
+    }
+    return fh_readlane(y, n);
+}
+
+// The fired weights of one step, compacted in edge order into `list` (a wavefront's own LDS, at least nch * 64 + 64
+// floats) and folded onto acc: per chunk one ballot, the lanes' prefix counts and one masked store, then fold_lanes
+// over the list 63 values at a time.
+template <int NC>
+__device__ __forceinline__ float step_fold(float acc, int k, int nch, const float* wc, const uint32_t* hist, float* list,
+                                           int lane) {
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < NC; ++u) {
+        if (u < nch) {
+            const bool fired = (hist[u] >> k) & 1u;
+            const uint64_t m = __builtin_amdgcn_ballot_w64(fired);
+            if (fired)
+                list[cnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+                    wc[u];
+            cnt += __builtin_popcountll(m);
+        }
+    }
+    for (int b0 = 0; b0 < cnt; b0 += 63) acc = fold_lanes(acc, list[b0 + lane], min(63, cnt - b0));
+    return acc;
+}
+
 // The in-edges [e0, e1) (at most NC * 64) of one cell, lanes over edges: presynaptic cell and weight per chunk;
 // lanes past the row end repeat its last entry (their bits are masked off by the caller).
 template <int NC>
@@ -73,6 +112,34 @@ __device__ __forceinline__ float chain_steps_bits(const Window& W, int e0, int e
         gk = lane == k ? acc : gk;
     }
     return gk;
+}
+
+// chain_steps_bits with each step's fired weights folded through `list` (step_fold): the same additions in the same
+// order.
+template <int NC>
+__device__ __forceinline__ float chain_steps_fold(const Window& W, int e0, int e1, float acc, int lane, const float* wc,
+                                                  const uint32_t* hist, float* list) {
+    const int nch = (e1 - e0 + kWave - 1) / kWave;
+    float gk = 0.0f;
+    for (int k = 0; k < W.K; ++k) {
+        acc = step_fold<NC>(acc * W.a_s, k, nch, wc, hist, list, lane);
+        gk = lane == k ? acc : gk;
+    }
+    return gk;
+}
+
+// The edges of a cell of at most NC * 64 in-edges in registers: weights and the window's bits of each presynaptic
+// history word (0 past the row end).
+template <int NC>
+__device__ __forceinline__ void load_window_edges(const Edges& a, const Window& W, int e0, int e1, int lane, float* wc,
+                                                  uint32_t* hist) {
+    int pre[NC];
+    load_edges<NC>(a, e0, e1, lane, pre, wc);
+#pragma unroll
+    for (int u = 0; u < NC; ++u) {
+        const uint64_t raw = a.hist[pre[u]];
+        hist[u] = e0 + u * kWave + lane < e1 ? W.bits(raw) : 0u;
+    }
 }
 
 // The chain of a cell of at most NC * 64 in-edges with its edges and history words held in registers.
@@ -189,9 +256,9 @@ __device__ __forceinline__ float g_chain_workgroup(const Edges& a, const Window&
 // chunks [wv * NCW, (wv + 1) * NCW) of the in-edges (at most 4 * NCW * 64; the caller streams longer cells) with
 // their window bits in registers.  For each step every wavefront compacts the weights of its edges that fired into
 // its own LDS list, in edge order (ballot + lane prefix count, one masked store per chunk); after a barrier
-// wavefront 0 folds the four lists in wavefront order onto g_{k-1} * a_s, which is edge order, the same additions as
-// the one-wavefront chain.  lists holds 4 * NCW * 64 floats, counts 4 ints; g_k ends in lane k of wavefront 0
-// (returned there; 0 elsewhere).
+// wavefront 0 folds the four lists in wavefront order onto g_{k-1} * a_s (fold_lanes), which is edge order, the same
+// additions as the one-wavefront chain.  counts holds 4 ints; g_k ends in lane k of wavefront 0
+// (returned there; 0 elsewhere).  lists: 4 * (NCW * 64 + 64) floats.
 template <int NCW>
 __device__ __forceinline__ float g_chain_compact(const Edges& a, const Window& W, int e0, int e1, float g0, int lane,
                                                  int wv, float* lists, int* counts) {
@@ -211,7 +278,8 @@ __device__ __forceinline__ float g_chain_compact(const Edges& a, const Window& W
 #pragma unroll
         for (int u = 0; u < NCW; ++u) hist[u] = pre[u] >= 0 ? W.bits(a.hist[pre[u]]) : 0u;
     }
-    float* const mine = lists + wv * NCW * kWave;
+    constexpr int kList = NCW * kWave + kWave;      // a wavefront's list, with room for the fold's last read
+    float* const mine = lists + wv * kList;
     float acc = g0, gk = 0.0f;
     for (int k = 0; k < W.K; ++k) {
         int cnt = 0;
@@ -234,12 +302,8 @@ __device__ __forceinline__ float g_chain_compact(const Edges& a, const Window& W
             acc = acc * W.a_s;
             for (int w = 0; w < 4; ++w) {
                 const int c = counts[w];
-                const float* list = lists + w * NCW * kWave;
-                for (int b0 = 0; b0 < c; b0 += kWave) {
-                    const float x = list[b0 + lane];          // read past the count only into this list's space
-                    const int nb = min(kWave, c - b0);
-                    for (int j = 0; j < nb; ++j) acc = acc + fh_readlane(x, j);
-                }
+                const float* list = lists + w * kList;
+                for (int b0 = 0; b0 < c; b0 += 63) acc = fold_lanes(acc, list[b0 + lane], min(63, c - b0));
             }
             gk = lane == k ? acc : gk;
         }

@@ -21,7 +21,6 @@ using fh_chain::kMaxWindow;
 using fh_chain::kShortEdges;
 using fh_chain::kSlots;
 using fh_chain::Window;
-constexpr int kVlongChunks = 36;        // chunks per wavefront for very long cells (up to 9216 in-edges in registers)
 
 enum { kPoisson = 1, kCounts = 4, kAdapt = 8 };
 
@@ -103,141 +102,92 @@ __device__ __forceinline__ void lif_window_cell(const Plain& a, uint32_t T0, int
     }
 }
 
-// One launch per window, three kinds of independent workgroups (nothing waits on anything):
-//   [0, n_vlong * batch)            one very long cell (more than 2048 in-edges) of one fly per workgroup: its chain
-//                                   over the four wavefronts (fh_chain::g_chain_compact: each step's fired weights
-//                                   compacted in parallel, folded in edge order by wavefront 0), then its LIF update;
-//   next ceil(long_items / 4)        one long cell (513-2048 in-edges) of one fly per wavefront: chain with the edges
-//                                   in registers, then its LIF update;
-//   the rest                         one block of kBlock consecutive cells of one fly per wavefront (blocks in `sched`
-//                                   order, most in-edges first).  Phase A: the block's short cells' in-edges form one
-//                                   contiguous range of the CSR; it is staged through LDS kSeg edges at a time (all
-//                                   index, weight and history loads of a segment in flight together), and the chain of
-//                                   each cell in the segment runs from LDS, g_k into LDS.  Phase B: lanes over the
-//                                   block's cells, the LIF updates (a cell without in-edges only decays: g_k =
-//                                   g_{k-1} * a_s).  Long cells inside a block are left to their own workgroups.
+// A window is one launch of independent work items (no item waits on another):
+//  long cells  one workgroup per (fly, cell with more than 512 in-edges): the cell's chain over the four wavefronts
+//              (fh_chain::g_chain_compact: each step's fired weights compacted in edge order by the wavefronts in
+//              parallel, folded by wavefront 0 with the lane fold), then its LIF update on wavefront 0;
+//  blocks      one wavefront per (fly, block of kBlock consecutive cells), blocks in `sched` order (most in-edges
+//              first).  Phase A: for each short cell of the block with in-edges, lanes over its in-edges (edges and
+//              window bits in registers, the register count fitted to the cell's chunks), the chain with each step's
+//              fired weights folded through a per-wavefront LDS list; g_k into LDS.  Phase B: lanes over the block's
+//              cells, the LIF updates (a cell without in-edges only decays: g_k = g_{k-1} * a_s).  Long cells are left
+//              to their own workgroups, which come first in the grid.
 // A cell's history word and state are written by the one wavefront that owns it; the bits other wavefronts read
 // from it (steps T0 - delay .. T0 - delay + K - 1) are not among those it rewrites (T0 .. T0 + K - 1).
 constexpr int kBlock = 16;
-constexpr int kSegChunks = 16;                 // a staged segment: 16 chunks of 64 in-edges
-constexpr int kSeg = kSegChunks * kWave;
 constexpr int kVlongCompact = 24;              // chunks per wavefront of a very long cell (up to 6144 in-edges)
+constexpr int kLongCompact = 8;               // chunks per wavefront of a long cell (up to 2048 in-edges)
 
-// Stage in-edges [s0, s1) (at most kSeg) of one fly into LDS: window bits and weights, one entry per edge.
-__device__ __forceinline__ void stage_segment(const Edges& E, const Window& W, int s0, int s1, int lane,
-                                              uint32_t* sbits, float* sw) {
-    int pre[kSegChunks];
-    float wv[kSegChunks];
-#pragma unroll
-    for (int u = 0; u < kSegChunks; ++u) {
-        const int e = s0 + u * kWave + lane;
-        const bool val = s0 + u * kWave < s1 && e < s1;
-        pre[u] = val ? E.idx[e] : -1;
-        wv[u] = val ? E.w[e] : 0.0f;
-    }
-#pragma unroll
-    for (int u = 0; u < kSegChunks; ++u) {
-        const uint32_t b = pre[u] >= 0 ? W.bits(E.hist[pre[u]]) : 0u;
-        if (s0 + u * kWave < s1) {
-            sbits[u * kWave + lane] = b;
-            sw[u * kWave + lane] = wv[u];
-        }
-    }
-}
-
-// The chain of one staged cell: its edges are entries [o0, o1) of the segment (at most 8 chunks).
-__device__ __forceinline__ float chain_from_lds(const Window& W, int o0, int o1, float g0, int lane,
-                                                const uint32_t* sbits, const float* sw) {
-    constexpr int NC = kShortEdges / kWave;
+// The chain of one short cell with its edges in registers, NC fitted to its chunk count.
+template <int NC>
+__device__ __forceinline__ float short_chain(const Edges& E, const Window& W, int e0, int e1, float g0, int lane,
+                                             float* list) {
     float wc[NC];
     uint32_t hist[NC];
-    const int nch = (o1 - o0 + kWave - 1) / kWave;
-#pragma unroll
-    for (int u = 0; u < NC; ++u) {
-        const int o = o0 + u * kWave + lane;
-        const bool val = u < nch && o < o1;
-        hist[u] = val ? sbits[o] : 0u;
-        wc[u] = val ? sw[o] : 0.0f;
-    }
-    return fh_chain::chain_steps_bits<NC>(W, o0, o1, g0, lane, wc, hist);
+    fh_chain::load_window_edges<NC>(E, W, e0, e1, lane, wc, hist);
+    return fh_chain::chain_steps_fold<NC>(W, e0, e1, g0, lane, wc, hist, list);
 }
 
-__global__ __launch_bounds__(256) void k_win(Plain a, const uint32_t* step_base, uint32_t step, int K, int vlong_wgs,
-                                             int long_wgs) {
-    __shared__ float gbuf[4][kBlock * kMaxWindow];
-    __shared__ uint32_t stage[4 * 2 * kSeg];       // short blocks: per wavefront kSeg bits + kSeg weights
+// The LDS of a workgroup: a long cell's four compaction lists, or a block wavefront's g_k table and fold list.
+constexpr int kLongLds = 4 * (kVlongCompact * kWave + kWave);
+constexpr int kBlockLds = kBlock * kMaxWindow + kShortEdges + kWave;
+static_assert(4 * kBlockLds <= kLongLds, "the block wavefronts' LDS fits in the long cells' buffer");
+
+// One launch per window: workgroups [0, long_wgs) take one (fly, long cell) each, the rest four blocks each.
+__global__ __launch_bounds__(256) void k_win(Plain a, const uint32_t* step_base, uint32_t step, int K, int long_wgs) {
+    __shared__ float lds[kLongLds];
     __shared__ int counts[4];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t T0 = step + (step_base ? *step_base : 0u);
     const Window W(T0, K, a.delay, a.a_s);
-    const int wg = blockIdx.x;
-    if (wg < vlong_wgs) {
-        const int fly = wg % a.batch, i = wg / a.batch;
+    if ((int)blockIdx.x < long_wgs) {
+        const int fly = blockIdx.x % a.batch, i = blockIdx.x / a.batch;
         const int r = a.long_rows[i];
         const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
         const float g0 = a.g[(int64_t)fly * a.n + r];
         const Edges E = edges_of(a, fly);
         float gk;
-        if (e1 - e0 <= 4 * kVlongCompact * kWave) {
-            gk = fh_chain::g_chain_compact<kVlongCompact>(E, W, e0, e1, g0, lane, wv, (float*)stage, counts);
-        } else {
+        if (e1 - e0 <= 4 * kLongCompact * kWave)
+            gk = fh_chain::g_chain_compact<kLongCompact>(E, W, e0, e1, g0, lane, wv, lds, counts);
+        else if (e1 - e0 <= 4 * kVlongCompact * kWave)
+            gk = fh_chain::g_chain_compact<kVlongCompact>(E, W, e0, e1, g0, lane, wv, lds, counts);
+        else
             gk = wv == 0 ? fh_chain::g_chain_stream(E, W, e0, e1, g0, lane) : 0.0f;
-        }
         if (wv == 0)
             lif_window_cell(a, T0, K, fly, r, g0, [&](int k, float) { return fh_readlane(gk, k); }, lane == 0);
         return;
     }
-    if (wg < vlong_wgs + long_wgs) {
-        const int64_t it = (int64_t)(wg - vlong_wgs) * 4 + wv;
-        if (it >= (int64_t)(a.n_long - a.n_vlong) * a.batch) return;
-        const int fly = (int)(it % a.batch), i = a.n_vlong + (int)(it / a.batch);
-        const int r = a.long_rows[i];
-        const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
-        const float g0 = a.g[(int64_t)fly * a.n + r];
-        const Edges E = edges_of(a, fly);
-        const float gk = e1 - e0 <= 16 * kWave ? fh_chain::g_chain_cached<16>(E, W, e0, e1, g0, lane)
-                                               : fh_chain::g_chain_cached<32>(E, W, e0, e1, g0, lane);
-        lif_window_cell(a, T0, K, fly, r, g0, [&](int k, float) { return fh_readlane(gk, k); }, lane == 0);
-        return;
-    }
-    const int64_t it = (int64_t)(wg - vlong_wgs - long_wgs) * 4 + wv;
+    const int64_t it = (int64_t)(blockIdx.x - long_wgs) * 4 + wv;
     const int blocks = (a.n + kBlock - 1) / kBlock;
     if (it >= (int64_t)blocks * a.batch) return;
     const int fly = (int)(it % a.batch);
     const int c0 = a.sched[it / a.batch] * kBlock;
     const int nc = min(kBlock, a.n - c0);
     const Edges E = edges_of(a, fly);
-    float* const gb = gbuf[wv];
-    uint32_t* const sbits = stage + wv * 2 * kSeg;
-    float* const sw = (float*)(sbits + kSeg);
-    const int ll = min(lane, nc);
-    const int pl0 = a.ptr[c0 + ll], pl1 = a.ptr[c0 + min(lane + 1, nc)];
+    float* const gb = lds + wv * kBlockLds;
+    float* const list = gb + kBlock * kMaxWindow;
+    const int pl0 = a.ptr[c0 + min(lane, nc)], pl1 = a.ptr[c0 + min(lane + 1, nc)];
     const bool is_long = lane < nc && pl1 - pl0 > kShortEdges;
     const float gl = lane < nc ? a.g[(int64_t)fly * a.n + c0 + lane] : 0.0f;
     // ---------------------------------------------------------------- phase A: chains of the short cells
-    // cells with in-edges that are not long, in order; segments of consecutive such cells of at most kSeg edges
     uint64_t todo = __builtin_amdgcn_ballot_w64(lane < nc && pl1 > pl0 && !is_long);
     while (todo) {
-        const int i0 = __builtin_ctzll(todo);
-        const int s0 = __builtin_amdgcn_readlane(pl0, i0);
-        // the segment: cells i0, i0 + 1, ... up to the first long cell or the first that ends beyond kSeg edges
-        // (cells without in-edges inside it add nothing to the range)
-        const uint64_t fits = __builtin_amdgcn_ballot_w64(lane >= i0 && lane < nc && pl1 - s0 <= kSeg && !is_long);
-        const uint64_t brk = ~fits & ~((1ull << i0) - 1);
-        const int i1 = brk ? __builtin_ctzll(brk) : 64;
-        const int last = min(i1, nc) - 1;
-        const int s1 = __builtin_amdgcn_readlane(pl1, last);
-        stage_segment(E, W, s0, s1, lane, sbits, sw);
-        __builtin_amdgcn_wave_barrier();
-        for (int i = i0; i <= last; ++i) {
-            const int e0 = __builtin_amdgcn_readlane(pl0, i), e1 = __builtin_amdgcn_readlane(pl1, i);
-            if (e1 == e0) continue;
-            const float gk = chain_from_lds(W, e0 - s0, e1 - s0, fh_readlane(gl, i), lane, sbits, sw);
-            if (lane < K) gb[i * kMaxWindow + lane] = gk;
-        }
-        __builtin_amdgcn_wave_barrier();
-        todo &= ~((last + 1 < 64 ? (1ull << (last + 1)) : 0ull) - 1);
+        const int i = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        const int e0 = __builtin_amdgcn_readlane(pl0, i), e1 = __builtin_amdgcn_readlane(pl1, i);
+        const float g0 = fh_readlane(gl, i);
+        const int nch = (e1 - e0 + kWave - 1) / kWave;
+        float gk;
+        if (nch <= 1)
+            gk = short_chain<1>(E, W, e0, e1, g0, lane, list);
+        else if (nch <= 2)
+            gk = short_chain<2>(E, W, e0, e1, g0, lane, list);
+        else if (nch <= 4)
+            gk = short_chain<4>(E, W, e0, e1, g0, lane, list);
+        else
+            gk = short_chain<8>(E, W, e0, e1, g0, lane, list);
+        if (lane < K) gb[i * kMaxWindow + lane] = gk;
     }
     __builtin_amdgcn_wave_barrier();
     // ---------------------------------------------------------------- phase B: LIF, lane = cell c0 + lane
@@ -278,12 +228,11 @@ FH_EXPORT int launch_lif_window(void** p, const int* dims, uint64_t seed, float 
     if (a.n_long > 0 && !a.long_rows) return FH_ERR_ARGS;
     if ((int64_t)a.n * a.batch == 0) return 0;
     if ((int64_t)a.n * a.batch >= (1ll << 31)) return FH_ERR_SIZE;
-    const int64_t vlong_wgs = (int64_t)a.n_vlong * a.batch;
-    const int64_t long_wgs = ((int64_t)(a.n_long - a.n_vlong) * a.batch + 3) / 4;
+    const int64_t long_wgs = (int64_t)a.n_long * a.batch;
     const int64_t block_wgs = (((int64_t)(a.n + kBlock - 1) / kBlock) * a.batch + 3) / 4;
-    const int64_t wgs = vlong_wgs + long_wgs + block_wgs;
-    if (wgs >= (1ll << 31)) return FH_ERR_SIZE;
-    hipLaunchKernelGGL(k_win, dim3((unsigned)wgs), dim3(256), 0, (hipStream_t)stream, a, step_base, step, K,
-                       (int)vlong_wgs, (int)long_wgs);
+    if (long_wgs >= (1ll << 31) || block_wgs >= (1ll << 31)) return FH_ERR_SIZE;
+    if (long_wgs + block_wgs >= (1ll << 31)) return FH_ERR_SIZE;
+    hipLaunchKernelGGL(k_win, dim3((unsigned)(long_wgs + block_wgs)), dim3(256), 0, (hipStream_t)stream, a, step_base,
+                       step, K, (int)long_wgs);
     FH_LAUNCH_CHECK();
 }
