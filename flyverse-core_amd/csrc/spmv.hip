@@ -6,6 +6,8 @@
 // order (torch sparse mm) is not pinned by anything it ships; agreement with it is to 1e-5 relative.
 #include "common.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int kChunk = 16;
@@ -308,4 +310,56 @@ FH_EXPORT int launch_spmv_chunked(void** p, int rows, int cols, int nchunks, int
     hipLaunchKernelGGL(k_chunk_fold, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        (const int*)p[2], (const float*)p[7], (float*)p[6], rows, nchunks, batch, scale, accumulate);
     FH_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ allocation-free variant
+// launch_spmv_chunked keeps its chunk partial sums (batch * nchunks floats) in a workspace the caller passes.  For
+// callers that cannot allocate (code under HIP graph capture, tight serving loops), launch_spmv_chunked_noalloc keeps
+// them in `scratch`, a file-scope device array of at most kNoAllocFloats floats: each GPU the library runs on holds
+// its own copy until the process ends.  Its device address on a GPU comes from hipGetSymbolAddress, taken once per
+// device, with that device current, on the entry point's first call there, and kept in g_scratch; host code passes
+// only that address to kernels.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU (on
+// different streams) race on it.  The chunk kernel writes every partial the fold then reads, so nothing in the array
+// needs clearing between calls.
+constexpr int64_t kNoAllocFloats = int64_t(1) << 22;    // 4,194,304 floats (16 MiB) per GPU
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace {
+std::mutex g_scratch_lock;
+float* g_scratch[kMaxDevices];          // device address of `scratch` on each GPU
+
+// The address of `scratch` on the current device (hipGetSymbolAddress on the first call there).
+int scratch_address(float** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kMaxDevices) return FH_ERR_ARGS;
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!ptr) return FH_ERR_ARGS;
+        g_scratch[dev] = (float*)ptr;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+}  // namespace
+
+// launch_spmv_chunked without the workspace: p = [ptr, chunk_row(nchunks), chunk_ptr(rows+1), idx, w, x (batch, cols),
+// y (batch, rows)]; the partial sums go to the library's device scratch.  Returns FH_ERR_SIZE when batch * nchunks
+// exceeds kNoAllocFloats.  The stream must belong to the current device.  Allocates nothing.
+FH_EXPORT int launch_spmv_chunked_noalloc(void** p, int rows, int cols, int nchunks, int batch, float scale,
+                                          int accumulate, void* stream) {
+    if (!p || rows < 0 || cols < 0 || batch < 0 || nchunks < 0) return FH_ERR_ARGS;
+    if ((int64_t)nchunks * batch > kNoAllocFloats) return FH_ERR_SIZE;
+    if ((int64_t)rows * batch == 0) return 0;
+    float* partial = nullptr;
+    const int err = scratch_address(&partial);
+    if (err) return err;
+    void* q[8] = {p[0], p[1], p[2], p[3], p[4], p[5], p[6], partial};
+    return launch_spmv_chunked(q, rows, cols, nchunks, batch, scale, accumulate, stream);
 }

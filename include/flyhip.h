@@ -73,6 +73,15 @@ int launch_pull(void** p, int n, int batch, float a_s, int flags, void* stream);
 int launch_spmv_chunked(void** p, int rows, int cols, int nchunks, int batch, float scale, int accumulate,
                         void* stream);
 
+/* launch_spmv_chunked without the workspace argument, for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); it has no counterpart in the reference.  The chunk partial
+ * sums go to a device array of the library itself (4,194,304 floats per GPU, held until the process ends), so
+ * batch * nchunks must not exceed 4,194,304 (FH_ERR_SIZE otherwise).  Calls that overlap on one GPU share that array
+ * and race.  p = [ptr, chunk_row(nchunks), chunk_ptr(rows+1), idx, w, x (batch, cols), y (batch, rows)]; the stream
+ * belongs to the current device.  Allocates nothing. */
+int launch_spmv_chunked_noalloc(void** p, int rows, int cols, int nchunks, int batch, float scale, int accumulate,
+                                void* stream);
+
 /* The launch_csr product in one pass over chunk-padded storage (row r's entries start at 16 * chunk_ptr[r], pads
  * have idx -1 and add nothing): one wavefront per row group (consecutive rows of at most 64 chunks and 64 rows
  * together, or one longer row) and fly; same additions in the same order as launch_csr.
