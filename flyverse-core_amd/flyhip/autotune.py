@@ -33,9 +33,41 @@ def _save(table: dict) -> None:
         pass
 
 
+def _device_tag(device) -> str:
+    dev = torch.cuda.get_device_properties(device)
+    return f"{dev.name}:{dev.multi_processor_count}"
+
+
 def batch_key(bb) -> str:
-    dev = torch.cuda.get_device_properties(bb.device)
-    return f"batch:{dev.name}:{dev.multi_processor_count}:rows{bb.n_rows}:nnz{bb.W_lif.nnz}:rate{bb.n_rate}:D{bb.n_delay}"
+    """Cache key of a batched engine's tuning: the device, the graph's shape and the spike ring's layout (fold mode 6
+    exists only on the slot-major ring)."""
+    return (f"batch:{_device_tag(bb.device)}:rows{bb.n_rows}:nnz{bb.W_lif.nnz}:rate{bb.n_rate}:D{bb.n_delay}"
+            f":ring-{bb.ring_layout}")
+
+
+def candidate_shapes(fold_modes, steps, grids, n_delay: int, ring_layout: str) -> list:
+    """The (fold mode, steps per launch, grid per CU) triples the engine accepts (batch.check_launch_shape)."""
+    from .batch import check_launch_shape
+    out = []
+    for mode, K, gpc in itertools.product(fold_modes, steps, grids):
+        try:
+            check_launch_shape(mode, K, n_delay, ring_layout)
+        except (ValueError, TypeError):
+            continue
+        out.append({"fold_mode": mode, "steps_per_launch": K, "grid_per_cu": gpc})
+    return out
+
+
+def cached_config(table: dict, key: str, n_delay: int, ring_layout: str):
+    """The cached winner for `key`, or None when there is none or the engine would refuse it."""
+    entry = table.get(key)
+    if entry is None:
+        return None
+    cfg = entry.get("config", {})
+    if not candidate_shapes([cfg.get("fold_mode")], [cfg.get("steps_per_launch")], [cfg.get("grid_per_cu")], n_delay,
+                            ring_layout):
+        return None
+    return entry
 
 
 def tune_batch(bb, frames: int = 5, fold_modes=None, steps=(4, 5, 6), grids=(64, 96, 128), use_cache=True,
@@ -43,23 +75,27 @@ def tune_batch(bb, frames: int = 5, fold_modes=None, steps=(4, 5, 6), grids=(64,
     """Time each (fold mode, steps per launch, grid per CU) on `bb` (a BatchBrain with rollouts queued) and leave
     it configured with the fastest.  `warm` frames run first: flies start at rest and their firing (the work per
     step) rises over the first tens of frames, so trials compare at a settled activity.  Returns {"config": ...,
-    "ms_per_frame": ..., "trials": [...]}."""
+    "ms_per_frame": ..., "trials": [...]}.  Raises ValueError when no shape of the search space fits the engine (for
+    example every steps-per-launch above its delay)."""
     if fold_modes is None:               # mode 6 needs the slot-major ring; mode 5 is the fastest without it
         fold_modes = (6,) if bb.ring_layout == "slot" else (5,)
+    shapes = candidate_shapes(fold_modes, steps, grids, bb.n_delay, bb.ring_layout)
+    if not shapes:
+        raise ValueError(f"no launch shape to try: fold modes {tuple(fold_modes)}, steps {tuple(steps)} and grids "
+                         f"{tuple(grids)} give none the engine accepts (delay {bb.n_delay} steps, "
+                         f"{bb.ring_layout}-major ring)")
     key = batch_key(bb)
     table = _load() if use_cache else {}
-    if key in table:
-        cfg = table[key]["config"]
-        bb.configure(**cfg)
-        return table[key] | {"cached": True}
+    hit = cached_config(table, key, bb.n_delay, bb.ring_layout)
+    if hit is not None:
+        bb.configure(**hit["config"])
+        return hit | {"cached": True}
     trials = []
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     if warm:
         bb.run(warm)
-    for mode, K, gpc in itertools.product(fold_modes, steps, grids):
-        if K > bb.n_delay or (mode == 6 and (bb.ring_layout != "slot" or K > 8)):
-            continue
-        bb.configure(fold_mode=mode, steps_per_launch=K, grid_per_cu=gpc)
+    for shape in shapes:
+        bb.configure(**shape)
         bb.run(1)                                   # capture + first replay
         torch.cuda.synchronize()
         e0.record()
@@ -67,9 +103,10 @@ def tune_batch(bb, frames: int = 5, fold_modes=None, steps=(4, 5, 6), grids=(64,
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / frames
-        trials.append({"fold_mode": mode, "steps_per_launch": K, "grid_per_cu": gpc, "ms_per_frame": round(ms, 4)})
+        trials.append(shape | {"ms_per_frame": round(ms, 4)})
         if log:
-            log(f"  tune: fold {mode} K {K} grid x{gpc}: {ms:.3f} ms/frame")
+            log(f"  tune: fold {shape['fold_mode']} K {shape['steps_per_launch']} grid x{shape['grid_per_cu']}: "
+                f"{ms:.3f} ms/frame")
     best = min(trials, key=lambda t: t["ms_per_frame"])
     cfg = {k: best[k] for k in ("fold_mode", "steps_per_launch", "grid_per_cu")}
     bb.configure(**cfg)
@@ -81,8 +118,7 @@ def tune_batch(bb, frames: int = 5, fold_modes=None, steps=(4, 5, 6), grids=(64,
 
 
 def packed_key(pb) -> str:
-    dev = torch.cuda.get_device_properties(pb.device)
-    return f"packed:{dev.name}:{dev.multi_processor_count}:n{pb.n}:nnz{pb.nnz}:k{pb.k}:D{pb.n_delay}"
+    return f"packed:{_device_tag(pb.device)}:n{pb.n}:nnz{pb.nnz}:k{pb.k}:D{pb.n_delay}"
 
 
 def tune_packed(pb, steps: int = 180, windows=None, grids=None, use_cache=True, log=None) -> dict:

@@ -338,3 +338,26 @@ def test_batch_launch_shapes():
                 (6, 9, 18, "slot"), (6, 6, 18, "row")]:
         with pytest.raises(ValueError):
             check_launch_shape(*bad)
+
+
+def test_tuner_cache_key_ring_layout_and_refused_shapes(monkeypatch):
+    """The batched tuner's cache key names the spike ring's layout; a cached shape the engine would refuse (fold mode
+    6 on a row-major ring) is not used; a search space with no acceptable shape is an error, raised before any GPU
+    work."""
+    from types import SimpleNamespace
+    from flyhip import autotune
+    monkeypatch.setattr(autotune, "_device_tag", lambda device: "testgpu:256")
+    fake = lambda ring: SimpleNamespace(device="cuda", n_rows=10, W_lif=SimpleNamespace(nnz=5), n_rate=3, n_delay=18,
+                                        ring_layout=ring)
+    slot, row = fake("slot"), fake("row")
+    assert autotune.batch_key(slot) != autotune.batch_key(row) and autotune.batch_key(row).endswith("ring-row")
+    table = {autotune.batch_key(row): {"config": {"fold_mode": 6, "steps_per_launch": 6, "grid_per_cu": 128}}}
+    assert autotune.cached_config(table, autotune.batch_key(row), 18, "row") is None
+    table[autotune.batch_key(slot)] = {"config": {"fold_mode": 6, "steps_per_launch": 6, "grid_per_cu": 128}}
+    assert autotune.cached_config(table, autotune.batch_key(slot), 18, "slot") is not None
+    assert autotune.cached_config({"k": {"config": {}}}, "k", 18, "slot") is None
+    shapes = autotune.candidate_shapes((5, 6), (4, 9), (64,), 18, "slot")
+    assert {(s_["fold_mode"], s_["steps_per_launch"]) for s_ in shapes} == {(5, 4), (5, 9), (6, 4)}
+    small = SimpleNamespace(**vars(fake("slot")) | {"n_delay": 3})
+    with pytest.raises(ValueError, match="no launch shape to try"):
+        autotune.tune_batch(small, steps=(4, 5, 6), use_cache=False)

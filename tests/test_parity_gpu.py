@@ -478,6 +478,26 @@ def test_fly_state_dict_round_trip(graphs):
     other = dict(state, body_ids=state["body_ids"] + 1)
     with pytest.raises(ValueError, match="connectome does not match"):
         b.load_state_dict(other)
+    # another seed: the checkpoint's Poisson key comes with the state, so the run continues as the original
+    c = FlyBrain(conn, retina, batch=2, device=DEV, seed=99, lif_params=lp, optic_params=op, cuda_graphs=graphs)
+    c.set_poisson(idx, hz)
+    c.random_movie(23)
+    c.load_state_dict(state)
+    assert c.brain.seed == 4
+    for _ in range(2):
+        c.step(10.0)
+    for k in ("v", "g", "spikes", "spike_counts"):
+        assert torch.equal(getattr(a.brain, k), getattr(c.brain, k)), k
+    # other dynamics: refused
+    from dataclasses import replace
+    d = FlyBrain(conn, retina, batch=2, device=DEV, seed=4, lif_params=replace(lp, tau_m=lp.tau_m * 2),
+                 optic_params=op, cuda_graphs=graphs)
+    with pytest.raises(ValueError, match="simulation parameters do not match"):
+        d.load_state_dict(state)
+    e = FlyBrain(conn, retina, batch=2, device=DEV, seed=4, lif_params=lp, optic_params=replace(op, gain_rr=0.5),
+                 cuda_graphs=graphs)
+    with pytest.raises(ValueError, match="simulation parameters do not match"):
+        e.load_state_dict(state)
 
 
 @pytest.mark.parametrize("graphs", [False, True])
