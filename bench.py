@@ -209,11 +209,7 @@ def packed_run(steps, log):
     from flyhip.brain import LIFParams
     from flyhip.packed import PackedBrains
     t0 = time.perf_counter()
-    mats = synth.packed_subgraphs(3, 128, 5_000, 80_000, mean_in_degree=153.0)
-    rng = np.random.default_rng(12)
-    inputs = [(np.sort(rng.choice(m.shape[0], 300, replace=False)), rng.uniform(0, 200, 300).astype(np.float32))
-              for m in mats]
-    seeds = rng.integers(1, 2**62, len(mats), dtype=np.uint64)
+    mats, inputs, seeds = synth.packed_workload()
     lp = LIFParams.shiu(dt=0.1)
     pb = PackedBrains(mats, lp, seeds=seeds, inputs=inputs, device="cuda")
     del mats

@@ -14,6 +14,7 @@ Workloads:
                     U[0, 200] Hz.
   packed_subgraphs  lesion / region-extraction stand-ins: LIF graphs of log-uniform size with the fly graph's LIF
                     block statistics.
+  packed_workload   those subgraphs with per-instance Philox seeds and 300 Poisson inputs at U[0, 200] Hz each.
 """
 from __future__ import annotations
 
@@ -172,17 +173,36 @@ def fly_inputs(seed: int, layout: FlyLayout, n_inputs: int = 300, hz_max: float 
 
 
 def packed_subgraphs(seed: int = 3, count: int = 128, n_min: int = 5_000, n_max: int = 80_000,
-                     mean_in_degree: float = 167.0):
+                     mean_in_degree: float = 167.0, which=None):
     """`count` LIF subgraphs with sizes log-uniform in [n_min, n_max] and the fly LIF block's statistics (in-degree
-    lognormal sigma 1 around `mean_in_degree`, power-law counts, 70 % excitatory presynaptic cells).  Returns the list
-    of per-instance (post, pre) signed-count CSR matrices."""
-    rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(math.log(n_min), math.log(n_max), count)).astype(np.int64)
+    lognormal sigma 1 around `mean_in_degree`, power-law counts, 70 % excitatory presynaptic cells).  Sizes come from
+    default_rng(seed), instance k's edges from default_rng([seed, k]), so any instance can be built alone.  Returns
+    the list of per-instance (post, pre) signed-count CSR matrices, of instances `which` (default: all)."""
+    sizes = np.exp(np.random.default_rng(seed).uniform(math.log(n_min), math.log(n_max), count)).astype(np.int64)
     mats = []
-    for nk in sizes:
-        nk = int(nk)
+    for k in (range(count) if which is None else which):
+        rng = np.random.default_rng([seed, k])
+        nk = int(sizes[k])
         cells = np.arange(nk)
         sign = np.where(rng.random(nk) < 0.7, 1.0, -1.0).astype(np.float32)
         r, c = _block(rng, cells, cells, int(mean_in_degree * nk))
-        mats.append(_assemble(nk, r, c, lambda k: power_law_counts(rng, k), sign))
+        mats.append(_assemble(nk, r, c, lambda m: power_law_counts(rng, m), sign))
     return mats
+
+
+def packed_workload(seed: int = 3, count: int = 128, n_min: int = 5_000, n_max: int = 80_000,
+                    mean_in_degree: float = 153.0, input_seed: int = 12, n_inputs: int = 300, max_hz: float = 200.0,
+                    which=None):
+    """The uneven-size batch: `count` subgraphs of log-uniform size (packed_subgraphs), each with its own Philox seed
+    and `n_inputs` distinct Poisson input cells at U[0, max_hz] Hz (drawn from default_rng([input_seed, k]) for
+    instance k).  Returns (mats, inputs, seeds) for instances `which` (default: all): inputs[j] = (sorted cell
+    indices, float32 rates), seeds a uint64 array."""
+    ks = list(range(count) if which is None else which)
+    mats = packed_subgraphs(seed, count, n_min, n_max, mean_in_degree=mean_in_degree, which=ks)
+    inputs, seeds = [], []
+    for k, m in zip(ks, mats):
+        rng = np.random.default_rng([input_seed, k])
+        inputs.append((np.sort(rng.choice(m.shape[0], n_inputs, replace=False)),
+                       rng.uniform(0, max_hz, n_inputs).astype(np.float32)))
+        seeds.append(rng.integers(1, 2**62, dtype=np.uint64))
+    return mats, inputs, np.array(seeds, np.uint64)

@@ -62,6 +62,7 @@ def lib():
         L.orc_uniform1.argtypes = [_U64, _U32, _U32, _U32, _U32]
         L.orc_uniform1.restype = ct.c_float
         L.orc_uniform.argtypes = [_f32p, _I, _I, _U64, _U32, _U32]
+        L.orc_uniform_at.argtypes = [_f32p, _i32p, _I, _U64, _U32, _U32, _U32]
         L.orc_lif.argtypes = [_I, _I, _I] + [_f32p] * 16
         L.orc_scale.argtypes = [_f32p, _L, _F]
         L.orc_events.argtypes = [_I, _I, _i64p, _i32p, _f32p, _f32p, _f32p]
@@ -91,6 +92,14 @@ def philox4x32_10(ctr, key):
 def uniform(batch, n, seed, step, stream=0):
     out = np.empty((batch, n), np.float32)
     lib().orc_uniform(out, batch, n, int(seed) & (2**64 - 1), int(step) & 0xFFFFFFFF, stream)
+    return out
+
+
+def uniform_at(idx, seed, step, fly=0, stream=0):
+    """The uniforms of cells idx only (the same values uniform() gives at those cells)."""
+    idx = np.ascontiguousarray(idx, np.int32)
+    out = np.empty(len(idx), np.float32)
+    lib().orc_uniform_at(out, idx, len(idx), int(seed) & (2**64 - 1), int(step) & 0xFFFFFFFF, fly, stream)
     return out
 
 
@@ -317,14 +326,27 @@ class FlyModel:
 
     flags = 32                                    # frozen-at-rest skip, as Brain(cuda_compact=True)
 
+    poisson_cells = None                          # set: draw uniforms only at these cells (the others get 1.0)
+
+    def _poisson_uniforms(self, B, N, step):
+        if self.poisson_cells is None:
+            return _per_fly_uniform(B, N, self.seeds, step, 0)
+        rnd = np.ones((B, N), np.float32)              # never below a probability <= 1: those cells cannot be forced
+        for b in range(B):
+            seed, fly = (int(self.seeds), b) if np.isscalar(self.seeds) else (int(self.seeds[b]), 0)
+            rnd[b, self.poisson_cells] = uniform_at(self.poisson_cells, seed, step, fly)
+        return rnd
+
     def lif_steps(self, steps, poisson_on=None, record=True):
         st = self.st
         on = bool((st.poisson_p > 0).any()) if poisson_on is None else poisson_on
         B, N = st.v.shape
+        if on and self.poisson_cells is not None and (st.poisson_p[:, np.setdiff1d(np.arange(N), self.poisson_cells)] > 0).any():
+            raise ValueError("poisson_cells must hold every cell with a Poisson probability")
         for _ in range(steps):
             scale(st.g, self.a_s)
             events(*self.csc, st.spike_buf[st.buf_pos], st.g)
-            rnd = _per_fly_uniform(B, N, self.seeds, st.step_count, 0) if on else st.poisson_p
+            rnd = self._poisson_uniforms(B, N, st.step_count) if on else st.poisson_p
             slot = st.spike_buf[st.buf_pos]
             d = dict(v=st.v, g=st.g, drive=st.drive, adapt=st.adapt, refrac=st.refrac, active=st.active,
                      poisson_p=st.poisson_p, res=st.res, std_u=st.std_u, spikes=st.spikes, out=slot, rate=st.rate,

@@ -49,6 +49,12 @@ float orc_uniform1(uint64_t seed, uint32_t step, uint32_t neuron, uint32_t fly, 
     return u01(o[0]);
 }
 
+/* out[j] = the uniform of cell idx[j] (counter (idx[j], step, fly, stream)): the draws a Poisson step can use, since
+ * a uniform in [0, 1) is never below a zero probability. */
+void orc_uniform_at(float* out, const int32_t* idx, int m, uint64_t seed, uint32_t step, uint32_t fly, uint32_t stream) {
+    for (int j = 0; j < m; ++j) out[j] = orc_uniform1(seed, step, (uint32_t)idx[j], fly, stream);
+}
+
 void orc_uniform(float* out, int batch, int n, uint64_t seed, uint32_t step, uint32_t stream) {
     for (int b = 0; b < batch; ++b)
         for (int i = 0; i < n; ++i) out[(int64_t)b * n + i] = orc_uniform1(seed, step, (uint32_t)i, (uint32_t)b, stream);

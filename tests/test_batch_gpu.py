@@ -105,3 +105,56 @@ def test_full_scale_batch_is_deterministic_and_matches_plain_path():
                    ("spike_counts", fb.brain.spike_counts)):
         assert np.array_equal(t[0].cpu().numpy(), lane7[key]), key
     assert np.array_equal(fb.optic.v[0].cpu().numpy(), optic7["v"])
+
+
+def test_full_scale_uneven_rollouts_with_refill_equal_solo_runs():
+    """The benchmark's B=64 workload at full scale (rollouts of 2,000-10,000 steps, 300 Poisson inputs each, one
+    captured graph per frame, fold mode 6 with 6 steps per launch and 128 workgroups per CU): lane 0's rollout is
+    shortened to 2,000 steps, so after 20 frames the lane is reset and takes the first queued rollout.  Lane 0 (its
+    first rollout at frame 20, the refilled one at frame 25) and two lanes whose rollouts run throughout equal the
+    plain FlyBrain (batch=1) run of their rollout, bit for bit."""
+    import bench
+    from flyhip.brain import LIFParams
+    from flyhip.fly import FlyBrain
+    from flyhip.optic import OpticParams
+    conn, retina, layout = synth.fly_graph(1, 1.0)
+    lp = LIFParams.shiu(dt=0.1)
+    op = OpticParams(tau_by_type=conn.rate_tau, baseline_by_type=conn.rate_baseline, pair_gain=[])
+    rs = bench.make_rollouts(layout, 70, seed=11)
+    rs[0].steps = 2000
+    bb = BatchBrain(conn, retina, lp, op, device=DEV)
+    bb.submit(rs)
+    bb.prepare(graph=True)
+    bb.configure(fold_mode=6, steps_per_launch=6, grid_per_cu=128)
+
+    def solo(r, frames):
+        fb = FlyBrain(conn, retina, batch=1, device=DEV, seed=r.seed, lif_params=lp, optic_params=op, cuda_graphs=True)
+        fb.set_poisson(r.input_idx, r.input_hz)
+        fb.random_movie(r.movie_seed)
+        for _ in range(frames):
+            fb.step(10.0)
+        torch.cuda.synchronize()
+        out = {k: getattr(fb.brain, k)[0].cpu().numpy() for k in ("v", "g", "refrac", "rate", "spike_counts")}
+        out["optic_v"] = fb.optic.v[0].cpu().numpy()
+        del fb
+        return out
+
+    def same(lane_state, optic_state, ref, what):
+        for k in ("v", "g", "refrac", "rate", "spike_counts"):
+            assert np.array_equal(lane_state[k], ref[k]), f"{what}: {k}"
+        assert np.array_equal(optic_state["v"], ref["optic_v"]), f"{what}: optic v"
+
+    bb.run(20)
+    torch.cuda.synchronize()
+    assert bb.lane_rollout[0] is rs[0] and bb.lane_left[0] == 0
+    same(bb.lane_state(0), bb.optic_state(0), solo(rs[0], 20), "lane 0, first rollout")
+    bb.run(5)
+    torch.cuda.synchronize()
+    assert bb.lane_rollout[0] is rs[64] and bb.h_frame[0] == 5
+    same(bb.lane_state(0), bb.optic_state(0), solo(rs[64], 5), "lane 0, refilled")
+    steady = [l for l in range(1, 64) if rs[l].steps >= 2500][:2]
+    assert len(steady) == 2
+    for l in steady:
+        assert bb.lane_rollout[l] is rs[l]
+        same(bb.lane_state(l), bb.optic_state(l), solo(rs[l], 25), f"lane {l}")
+    assert bb.completed >= 1

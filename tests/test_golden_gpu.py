@@ -1,6 +1,9 @@
-"""The GPU path against the committed golden fixtures at the workloads' real sizes: the plain-LIF workload (16,384
-neurons, 1,000 steps) on Brain, and ten 10 ms frames of the full-scale fly (167,106 neurons, LIF + optic lobe) on
-FlyBrain with captured frames, both bit for bit (per-step or per-frame spike totals, raster / state sha256)."""
+"""The GPU path against the committed golden fixtures at the workloads' real sizes, bit for bit: the plain-LIF
+workload (16,384 neurons, 1,000 steps) on Brain (per-step spike totals, raster and state sha256); 100 frames of 10 ms
+(10,000 steps) of the full-scale fly (167,106 neurons, LIF + optic lobe) on FlyBrain with captured frames (per-frame
+spike totals, final LIF and optic state sha256); and the 128 uneven subgraphs of the packed workload stepped 5,000
+steps in one graph replay after a tuning pass, as the benchmark runs them (every instance's spike total and final
+state sha256 against its solo oracle run)."""
 import json
 from pathlib import Path
 
@@ -69,3 +72,27 @@ def test_full_scale_fly_frames_match_golden():
     pr_cell = torch.as_tensor(retina.cells()["pr_cell"].astype(np.int64), device=o.I_lp.device)
     ost = [t[0].cpu().numpy() for t in (o.v, o.adapt, o.I_lp[:, pr_cell], o.I_mean[:, pr_cell])]
     assert mg.digest(*ost) == want["optic_sha256"]
+
+
+def test_packed_workload_matches_golden_as_the_benchmark_runs_it():
+    from flyhip.autotune import tune_packed
+    from flyhip.packed import PackedBrains
+    mg = _mg()
+    want = json.loads((GOLD / "c3_packed.json").read_text())
+    C3 = want["config"]
+    mats, inputs, seeds = mg.c3_workload()
+    lp = LIFParams.shiu(dt=0.1)
+    pb = PackedBrains(mats, lp, seeds=seeds, inputs=inputs, device="cuda")
+    del mats
+    tune_packed(pb, use_cache=False)                  # leaves the fastest shape configured, back at step 0
+    pb.run(C3["steps"])
+    torch.cuda.synchronize()
+    pb.check()
+    bad = []
+    for k, ref in want["instances"].items():
+        st = pb.instance_state(int(k))
+        total = int(st["counts"].sum())
+        sha = mg.digest(st["v"], st["g"], st["refrac"], st["rate"], st["counts"])
+        if total != ref["count_totals"][-1] or sha != ref["state_sha256"]:
+            bad.append(k)
+    assert not bad, f"{len(bad)} of {len(want['instances'])} instances differ, e.g. {bad[:5]}"
