@@ -59,6 +59,7 @@ def lib():
             build()
         L = ct.CDLL(str(LIB_PATH))
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_set_threads.argtypes = [_I]
         L.orc_uniform1.argtypes = [_U64, _U32, _U32, _U32, _U32]
         L.orc_uniform1.restype = ct.c_float
         L.orc_uniform.argtypes = [_f32p, _I, _I, _U64, _U32, _U32]
@@ -83,6 +84,11 @@ def _f32(a):
 
 
 # ------------------------------------------------------------------ primitives
+def set_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops (each writes disjoint elements: the results do not change)."""
+    lib().orc_set_threads(int(n))
+
+
 def philox4x32_10(ctr, key):
     out = np.zeros(4, np.uint32)
     lib().orc_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)

@@ -11,6 +11,19 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the parallel loops below (every one writes disjoint elements: per element, per row or per fly, so the
+ * results do not depend on the thread count). */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
 
 /* ------------------------------------------------------------------ Philox4x32-10
  * Counter-based RNG of Salmon et al., "Parallel random numbers: as easy as 1, 2, 3" (SC'11), the
@@ -56,6 +69,7 @@ void orc_uniform_at(float* out, const int32_t* idx, int m, uint64_t seed, uint32
 }
 
 void orc_uniform(float* out, int batch, int n, uint64_t seed, uint32_t step, uint32_t stream) {
+#pragma omp parallel for schedule(static)
     for (int b = 0; b < batch; ++b)
         for (int i = 0; i < n; ++i) out[(int64_t)b * n + i] = orc_uniform1(seed, step, (uint32_t)i, (uint32_t)b, stream);
 }
@@ -72,6 +86,7 @@ void orc_lif(int n, int batch, int flags, float* v, float* g, const float* drive
              const float* active, const float* prob, const float* rnd, float* res, const float* std_u,
              float* spikes, float* out, float* rate, float* counts, const float* P, const float* phase) {
     int64_t total = (int64_t)n * batch;
+#pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < total; ++t) {
         int i = (int)(t % n);
         if ((flags & 32) && active[i] == 0.0f && (!(flags & 1) || prob[t] == 0.0f) && v[t] == P[0] && g[t] == 0.0f &&
@@ -127,6 +142,7 @@ void orc_lif(int n, int batch, int flags, float* v, float* g, const float* drive
 
 /* g *= a_s (flyverse/brain.py:832) */
 void orc_scale(float* g, int64_t count, float a) {
+#pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < count; ++t) g[t] = g[t] * a;
 }
 
@@ -137,6 +153,7 @@ void orc_scale(float* g, int64_t count, float a) {
  * rounded before the add.  out_ptr/out_post/out_w are the pre-major CSC arrays of set_weights (brain.py:535-538). */
 void orc_events(int n, int batch, const int64_t* out_ptr, const int32_t* out_post, const float* out_w,
                 const float* x, float* g) {
+#pragma omp parallel for schedule(static)
     for (int b = 0; b < batch; ++b) {
         const float* xb = x + (int64_t)b * n;
         float* gb = g + (int64_t)b * n;
@@ -163,6 +180,7 @@ void orc_events(int n, int batch, const int64_t* out_ptr, const int32_t* out_pos
 static float orc_half_round(float x);
 void orc_spmv(int rows, int cols, int batch, const int64_t* ptr, const int32_t* idx, const float* w,
               const float* x, float* y, int half) {
+#pragma omp parallel for collapse(2) schedule(dynamic, 256)
     for (int b = 0; b < batch; ++b)
         for (int r = 0; r < rows; ++r) {
             float total = 0.0f;
@@ -203,6 +221,7 @@ static float orc_half_round(float x) {
 static inline float clip01(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
 
 void orc_optic_dr(int n, int batch, const float* v, const float* bvec, float* dr) {
+#pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < (int64_t)n * batch; ++t) {
         float b = bvec[t % n];
         float s = v[t] + b;
@@ -215,6 +234,7 @@ void orc_optic_dr(int n, int batch, const float* v, const float* bvec, float* dr
  *   v = inp + (v - inp)*a; adapt = dr + (adapt - dr)*a_ad; then the next dr (fused, neural.cu:78). */
 void orc_optic_update(int n, int batch, float* v, float* adapt, float* dr, const float* y, const float* pr,
                       const float* spk, const float* a, const float* bvec, float gain_rr, float adapt_gain, float a_ad) {
+#pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < (int64_t)n * batch; ++t) {
         int i = (int)(t % n);
         float old = dr[t];
@@ -250,6 +270,7 @@ void orc_photoreceptor(int n_col, int n_pr, int batch, const float* rad, const f
                        const int32_t* cell_ptr, const int32_t* cell_pr, const float* cell_w, float* I_lp,
                        float* I_mean, uint8_t* fresh, const float* coefs, float* contrast, float* a_pr) {
     const int n_cells = n_col * 5;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int b = 0; b < batch; ++b) {
         for (int k = 0; k < n_cells; ++k) {
             int64_t t = (int64_t)b * n_cells + k;
