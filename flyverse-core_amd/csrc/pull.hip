@@ -151,6 +151,66 @@ __global__ __launch_bounds__(64) void k_events(const int* __restrict__ ptr, cons
     for (int k = lane; k < hi - lo; k += kWave) gb[lo + k] = tile[k];
 }
 
+// The synaptic input of the clocked step (flyverse/brain.py:903-921): per (fly, row) wavefront, g = g * a_s[row]
+// (the row's decay this phase, 1 when its clock does not tick); then, when any clock class forms its product this
+// step, g = g + y with y the row's product if its class ticks (its in-edges' w * x_in[pre] for x_in != 0, folded from
+// +0 in ascending presynaptic order, x_in the class's input: the delayed spikes for clock 1, the class's
+// accumulated spikes otherwise) and +0 for the other rows (the zero rows of the other classes' products).
+constexpr int kMaxClockClasses = 8;
+struct ClockInputs {
+    const float* x[kMaxClockClasses];
+};
+
+__global__ __launch_bounds__(256) void k_pull_clocked(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                      const float* __restrict__ w, ClockInputs xin,
+                                                      const unsigned char* __restrict__ row_class,
+                                                      const float* __restrict__ a_s, float* __restrict__ g, int n,
+                                                      int batch, int tick_mask) {
+    const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (gw >= (int64_t)n * batch) return;
+    const int lane = threadIdx.x & 63;
+    const int b = (int)(gw / n), row = (int)(gw % n);
+    float acc = g[gw] * a_s[row];
+    if (tick_mask) {
+        const int cls = row_class[row];
+        float y = 0.0f;
+        if ((tick_mask >> cls) & 1) {
+            const float* xb = xin.x[cls] + (int64_t)b * n;
+            const int start = ptr[row], end = ptr[row + 1];
+            for (int base = start; base < end; base += kWave) {
+                const int e = base + lane;
+                bool act = false;
+                float c = 0.0f;
+                if (e < end) {
+                    const float xv = xb[idx[e]];
+                    if (xv != 0.0f) { act = true; c = w[e] * xv; }
+                }
+                uint64_t m = __builtin_amdgcn_ballot_w64(act);
+                while (m) {
+                    const int j = __builtin_ctzll(m);
+                    y = y + fh_readlane(c, j);
+                    m &= m - 1;
+                }
+            }
+        }
+        acc = acc + y;
+    }
+    if (lane == 0) g[gw] = acc;
+}
+
+// The clock accumulators (flyverse/brain.py:914-918): acc_j += x for the classes in add_mask, then acc_j = 0 for
+// those in clear_mask.
+__global__ __launch_bounds__(256) void k_clock_acc(const float* __restrict__ x, ClockInputs acc, int64_t count,
+                                                   int nacc, int add_mask, int clear_mask) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    for (int j = 0; j < nacc; ++j) {
+        float* a = const_cast<float*>(acc.x[j]);
+        if ((add_mask >> j) & 1) a[t] = a[t] + x[t];
+        if ((clear_mask >> j) & 1) a[t] = 0.0f;
+    }
+}
+
 }  // namespace
 
 // g[b, post] = g[b, post] * a_s (if flags & 2 is clear) + sum over in-edges in ascending presynaptic order of
@@ -197,5 +257,41 @@ FH_EXPORT int launch_events(void** p, int n, int batch, int half, int count, voi
     else
         hipLaunchKernelGGL(k_events<float>, grid, dim3(64), 0, (hipStream_t)stream, (const int*)p[0], (const int*)p[1],
                            (const float*)p[2], (const float*)p[3], (float*)p[4], (const int*)p[5], n, count);
+    FH_LAUNCH_CHECK();
+}
+
+// The clocked synaptic input.  p = [ptr (n+1), idx, w (float), g (batch, n), row_class uint8 (n: the row's clock
+// class), a_s (n: this phase's decay per row), x_0 .. x_{ncls-1} (batch, n): each class's input].  tick_mask: bit j
+// set when class j forms its product this step (0: decay only).  Replaces the g.mul_(c["a_s"]) and the _matmul_add
+// calls of Brain._step_clocked, flyverse/brain.py:905-921, with a deterministic product order.
+FH_EXPORT int launch_pull_clocked(void** p, int n, int batch, int ncls, int tick_mask, void* stream) {
+    if (!p || n < 0 || batch < 0 || ncls < 1 || ncls > kMaxClockClasses || (tick_mask >> ncls)) return FH_ERR_ARGS;
+    const int64_t waves = (int64_t)n * batch;
+    if (waves == 0) return 0;
+    if (waves * 64 >= (1ll << 40)) return FH_ERR_SIZE;
+    ClockInputs xin{};
+    for (int j = 0; j < ncls; ++j) {
+        xin.x[j] = (const float*)p[6 + j];
+        if (((tick_mask >> j) & 1) && !xin.x[j]) return FH_ERR_ARGS;
+    }
+    hipLaunchKernelGGL(k_pull_clocked, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
+                       (const int*)p[0], (const int*)p[1], (const float*)p[2], xin, (const unsigned char*)p[4],
+                       (const float*)p[5], (float*)p[3], n, batch, tick_mask);
+    FH_LAUNCH_CHECK();
+}
+
+// The clock accumulators: p = [x (count), acc_0 .. acc_{nacc-1} (count each)]; acc_j += x for j in add_mask, then
+// acc_j = 0 for j in clear_mask (flyverse/brain.py:914-918).
+FH_EXPORT int launch_clock_acc(void** p, int64_t count, int nacc, int add_mask, int clear_mask, void* stream) {
+    if (!p || count < 0 || nacc < 0 || nacc > kMaxClockClasses || (add_mask >> nacc) || (clear_mask >> nacc))
+        return FH_ERR_ARGS;
+    if (count == 0 || nacc == 0) return 0;
+    ClockInputs acc{};
+    for (int j = 0; j < nacc; ++j) {
+        acc.x[j] = (const float*)p[1 + j];
+        if (!acc.x[j]) return FH_ERR_ARGS;
+    }
+    hipLaunchKernelGGL(k_clock_acc, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)p[0], acc, count, nacc, add_mask, clear_mask);
     FH_LAUNCH_CHECK();
 }

@@ -66,6 +66,18 @@ int launch_scale(float* x, int64_t count, float a, void* stream);
  * p = [ptr(n+1), idx, w (float, or half if flags & 1), x (batch, n), g (batch, n)], post-major CSR. */
 int launch_pull(void** p, int n, int batch, float a_s, int flags, void* stream);
 
+/* The synaptic input of the clocked step (per-module clocks): g = g * a_s[row] (this phase's decay, 1 for rows whose
+ * clock does not tick), then, when tick_mask is nonzero, g = g + y with y the row's product over its clock class's
+ * input (w * x for x != 0 folded from +0 in ascending presynaptic order) if that class ticks, +0 otherwise.
+ * Replaces g.mul_(c["a_s"]) and the _matmul_add calls of Brain._step_clocked, flyverse/brain.py:905-921.
+ * p = [ptr (n+1), idx, w (float), g (batch, n), row_class uint8 (n), a_s (n), x_0 .. x_{ncls-1} (batch, n)];
+ * ncls <= 8; bit j of tick_mask: class j forms its product this step. */
+int launch_pull_clocked(void** p, int n, int batch, int ncls, int tick_mask, void* stream);
+
+/* The clock accumulators of the clocked step (flyverse/brain.py:914-918): acc_j += x for j in add_mask, then
+ * acc_j = 0 for j in clear_mask.  p = [x (count), acc_0 .. acc_{nacc-1} (count each)]; nacc <= 8. */
+int launch_clock_acc(void** p, int64_t count, int nacc, int add_mask, int clear_mask, void* stream);
+
 /* The launch_csr product through a precomputed chunk table (one lane per 16-entry chunk, then one lane per row),
  * y = scale * W x (+ y if accumulate).  Replaces the _mv products of flyverse/optic.py:161-165, 579-580, 602.
  * p = [ptr, chunk_row(nchunks), chunk_ptr(rows+1), idx, w, x (batch, cols), y (batch, rows), partial (batch,

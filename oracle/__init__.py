@@ -333,6 +333,60 @@ class FlyModel:
     flags = 32                                    # frozen-at-rest skip, as Brain(cuda_compact=True)
 
     poisson_cells = None                          # set: draw uniforms only at these cells (the others get 1.0)
+    clocks = None                                 # set by set_clocks
+
+    def set_clocks(self, k, W, p):
+        """Per-neuron clocks (flyverse/brain.py:618-664): k (N,) multipliers, W the (post, pre) weights, p the LIF
+        parameters.  The phase coefficients restate brain.py:630-645 with torch float32 CPU ops; each clock's rows of
+        W as a pre-major CSC (its products fold in ascending presynaptic order from +0); one accumulator per clock
+        longer than one step."""
+        import torch
+        import scipy.sparse as sp
+        k = np.asarray(k, np.int64)
+        K = int(np.lcm.reduce(np.unique(k)))
+        kt = torch.from_numpy(k.astype(np.float32))
+        std_on = p.std_u > 0
+        phases = []
+        for ph in range(K):
+            u = torch.from_numpy(((ph % k) == 0).astype(np.float32))
+            one = torch.ones_like(u)
+            coef = lambda tau: torch.where(u > 0, torch.exp(-p.dt * kt / tau), one)
+            rows = [coef(p.tau_m), u * kt * p.dt, coef(p.adapt_tau) if p.adapt_jump > 0 else torch.zeros_like(u),
+                    coef(p.std_tau) if std_on else one, coef(p.rate_tau),
+                    u * (1 - torch.exp(-p.dt * kt / p.rate_tau)) * 1000.0 / (kt * p.dt), u, u * kt]
+            phases.append((coef(p.tau_syn).numpy(), torch.stack(rows).numpy().astype(np.float32)))
+        cscs = {}
+        for kk in np.unique(k):
+            Wk = (sp.diags((k == kk).astype(np.float32)) @ sp.csr_matrix(W)).tocsc()
+            Wk.eliminate_zeros()
+            Wk.sort_indices()
+            cscs[int(kk)] = (Wk.indptr.astype(np.int64), Wk.indices.astype(np.int32), Wk.data.astype(np.float32))
+        B, N = self.st.v.shape
+        self.clocks = {"K": K, "phases": phases, "cscs": cscs,
+                       "acc": {int(kk): np.zeros((B, N), np.float32) for kk in np.unique(k) if kk > 1}}
+
+    def _clocked_input(self, x):
+        """The synaptic input of Brain._step_clocked (flyverse/brain.py:903-921): g *= a_s of the phase; for each clock
+        in ascending order, clock 1 adds W_1 x, a longer clock adds x to its accumulator and, on the steps its period
+        divides, adds W_k acc (then clears it); each product y from +0, then g + y."""
+        st, ck = self.st, self.clocks
+        a_s, lifphase = ck["phases"][st.step_count % ck["K"]]
+        st.g[:] = (st.g * a_s[None]).astype(np.float32)
+        for kk, csc in ck["cscs"].items():
+            if kk == 1:
+                xin = x
+            else:
+                acc = ck["acc"][kk]
+                acc += x
+                if st.step_count % kk:
+                    continue
+                xin = acc
+            y = np.zeros_like(st.g)
+            events(csc[0], csc[1], csc[2], xin, y)
+            st.g[:] = (st.g + y).astype(np.float32)
+            if kk > 1:
+                ck["acc"][kk][:] = 0.0
+        return lifphase
 
     def _poisson_uniforms(self, B, N, step):
         if self.poisson_cells is None:
@@ -350,14 +404,19 @@ class FlyModel:
         if on and self.poisson_cells is not None and (st.poisson_p[:, np.setdiff1d(np.arange(N), self.poisson_cells)] > 0).any():
             raise ValueError("poisson_cells must hold every cell with a Poisson probability")
         for _ in range(steps):
-            scale(st.g, self.a_s)
-            events(*self.csc, st.spike_buf[st.buf_pos], st.g)
+            phase = None
+            if self.clocks is None:
+                scale(st.g, self.a_s)
+                events(*self.csc, st.spike_buf[st.buf_pos], st.g)
+            else:
+                phase = self._clocked_input(st.spike_buf[st.buf_pos])
             rnd = self._poisson_uniforms(B, N, st.step_count) if on else st.poisson_p
             slot = st.spike_buf[st.buf_pos]
             d = dict(v=st.v, g=st.g, drive=st.drive, adapt=st.adapt, refrac=st.refrac, active=st.active,
                      poisson_p=st.poisson_p, res=st.res, std_u=st.std_u, spikes=st.spikes, out=slot, rate=st.rate,
                      spike_counts=st.spike_counts)
-            lif(d, self.flags | (1 if on else 0) | (4 if record else 0), self.P, rnd)
+            flags = self.flags | (1 if on else 0) | (4 if record else 0) | (16 if phase is not None else 0)
+            lif(d, flags, self.P, rnd, phase)
             st.buf_pos = (st.buf_pos + 1) % st.n_delay
             st.step_count += 1
             yield st

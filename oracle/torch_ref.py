@@ -120,6 +120,94 @@ class RefBrain:
         self.rate.mul_(self._a_r).add_(spikes * ((1 - self._a_r) * 1000.0 / p.dt))
 
 
+class RefClockedBrain(RefBrain):
+    """Brain._step_clocked on the torch branch (flyverse/brain.py:618-664, 898-957): per-neuron clocks with the phase
+    coefficients of set_clocks, one accumulator per clock longer than a step, and each clock's product W_k x formed
+    into zeros with index_add_ in ascending presynaptic order (the product order this build defines) before g.add_."""
+
+    def set_clocks(self, k, W):
+        p = self.p
+        k = np.asarray(k, np.int64)
+        self.K = int(np.lcm.reduce(np.unique(k)))
+        kt = torch.from_numpy(k.astype(np.float32))
+        self._phase = []
+        for ph in range(self.K):
+            u = torch.from_numpy(((ph % k) == 0).astype(np.float32))
+            one = torch.ones_like(u)
+            coef = lambda tau: torch.where(u > 0, torch.exp(-p.dt * kt / tau), one)
+            self._phase.append({"u": u, "a_m": coef(p.tau_m), "a_s": coef(p.tau_syn), "a_r": coef(p.rate_tau),
+                                "a_ad": coef(p.adapt_tau) if p.adapt_jump > 0 else torch.zeros_like(u),
+                                "a_std": coef(p.std_tau) if self._std_on else one, "dt": u * kt * p.dt,
+                                "rate_gain": u * (1 - torch.exp(-p.dt * kt / p.rate_tau)) * 1000.0 / (kt * p.dt),
+                                "pois": u * kt})
+        self._W_k, self._acc = {}, {}
+        for kk in np.unique(k):
+            Wk = sp.csc_matrix(sp.diags((k == kk).astype(np.float32)) @ sp.csr_matrix(W))
+            Wk.eliminate_zeros()
+            Wk.sort_indices()
+            self._W_k[int(kk)] = (torch.from_numpy(Wk.indptr.astype(np.int64)), torch.from_numpy(Wk.indices.astype(np.int64)),
+                                  torch.from_numpy(Wk.data.astype(np.float32)))
+            if kk > 1:
+                self._acc[int(kk)] = torch.zeros(self.B, self.n)
+        self.step_count = 0
+
+    def _product(self, Wk, x):
+        ptr, post, w = Wk
+        y = torch.zeros(self.B * self.n)
+        nz = torch.nonzero(x)
+        if nz.shape[0]:
+            bidx, pre = nz[:, 0], nz[:, 1]
+            start = ptr[pre]
+            cnt = ptr[pre + 1] - start
+            total = int(cnt.sum())
+            if total:
+                seg = torch.repeat_interleave(torch.arange(nz.shape[0]), cnt, output_size=total)
+                first = torch.repeat_interleave(torch.cumsum(cnt, 0) - cnt, cnt, output_size=total)
+                e = start[seg] + torch.arange(total) - first
+                y.index_add_(0, post[e] + bidx[seg] * self.n, w[e] * x[bidx[seg], pre[seg]])
+        return y.view(self.B, self.n)
+
+    def step(self, rnd=None):
+        p = self.p
+        c = self._phase[self.step_count % self.K]
+        x = self.spike_buf[self.buf_pos]
+        self.g.mul_(c["a_s"])
+        for kk, Wk in self._W_k.items():
+            if kk == 1:
+                self.g.add_(self._product(Wk, x))
+            else:
+                acc = self._acc[kk]
+                acc.add_(x)
+                if (self.step_count % kk) == 0:
+                    self.g.add_(self._product(Wk, acc))
+                    acc.zero_()
+        target = p.v_rest + self.g + self.drive - self.adapt
+        torch.add(target, (self.v - target) * c["a_m"], out=self.v)
+        in_ref = self.refrac > 0
+        torch.where(in_ref, torch.full_like(self.v, p.v_reset), self.v, out=self.v)
+        torch.clamp(self.refrac - c["dt"], min=0.0, out=self.refrac)
+        torch.mul((self.v >= p.v_th).float() * c["u"], self.active, out=self.spikes)
+        spikes = self.spikes
+        if rnd is not None:
+            forced = (rnd < self.poisson_p * c["pois"]).float()
+            torch.maximum(spikes, forced, out=spikes)
+        fired = spikes > 0
+        torch.where(fired, torch.full_like(self.v, p.v_reset), self.v, out=self.v)
+        torch.where(fired, torch.full_like(self.refrac, p.t_ref), self.refrac, out=self.refrac)
+        if p.adapt_jump > 0:
+            self.adapt.mul_(c["a_ad"]).add_(spikes, alpha=p.adapt_jump)
+        if self._std_on:
+            self.spike_buf[self.buf_pos] = spikes * self.res
+            torch.where(fired, self.res * (1 - self.std_u_vec), self.res, out=self.res)
+            torch.sub(1.0, (1.0 - self.res) * c["a_std"], out=self.res)
+        else:
+            self.spike_buf[self.buf_pos] = spikes
+        self.buf_pos = (self.buf_pos + 1) % self.n_delay
+        self.spike_counts.add_(spikes)
+        self.rate.mul_(c["a_r"]).add_(spikes * c["rate_gain"])
+        self.step_count += 1
+
+
 class RefOptic:
     """flyverse/optic.py's OpticLobe frame on the CPU branch (:496-604), torch ops, products in the chunked order."""
 

@@ -52,8 +52,10 @@ def oracle_fly(conn, retina, lp, op, B, seeds, movie_seeds, with_optic=True):
     P = oracle.lif_params_vector(lp)
     n_delay = max(1, int(round(lp.delay / lp.dt)))
     spec = oracle.OptSpec(m, op, retina.pr_sens, conn.n, retina.cells()) if m is not None else None
-    return oracle.FlyModel(conn.n, B, P, math.exp(-lp.dt / lp.tau_syn), n_delay, csc, active, seeds, spec,
-                           movie_seeds, v_rest=lp.v_rest, dt=lp.dt)
+    model = oracle.FlyModel(conn.n, B, P, math.exp(-lp.dt / lp.tau_syn), n_delay, csc, active, seeds, spec,
+                            movie_seeds, v_rest=lp.v_rest, dt=lp.dt)
+    model.W = W                                   # the (post, pre) weights the model steps with
+    return model
 
 
 def grouped_retina(retina, per_column=3, seed=0):

@@ -21,7 +21,7 @@ import oracle
 from flyhip.brain import shaped_weights
 from flyhip.optic import optic_matrices
 from helpers import fly_setup, oracle_fly
-from oracle.torch_ref import RefBrain, RefOptic, chunked_mv, fly_frame_steps
+from oracle.torch_ref import RefBrain, RefClockedBrain, RefOptic, chunked_mv, fly_frame_steps
 
 
 def test_chunked_product_agrees_with_torch_sparse_product():
@@ -117,4 +117,34 @@ def test_brain_step_oracle_equals_the_reference_torch_step(adapt, std):
         for name in ("v", "g", "refrac", "rate", "spike_counts", "adapt", "res"):
             assert torch.equal(getattr(rb, name), torch.from_numpy(getattr(st, name))), (step, name)
         assert torch.equal(rb.spike_buf, torch.from_numpy(st.spike_buf))
+    assert fired > 100
+
+
+@pytest.mark.parametrize("adapt", [0.0, 1.5])
+def test_clocked_steps_oracle_equals_the_reference_torch_step(adapt):
+    """The oracle's per-module clocks (FlyModel.set_clocks: phase coefficients, accumulators, the clocks' products
+    and the fused update with flag 16) against Brain._step_clocked's torch branch (flyverse/brain.py:898-957) op by
+    op, with clocks of 1, 2, 3 and 6 steps (period 6) and Poisson forcing."""
+    from flyhip import synth
+    from flyhip.brain import LIFParams
+    c, inputs, hz = synth.plain_lif_graph(6, n=700, n_inputs=80, hz=400.0)
+    lp = LIFParams.shiu(dt=0.1, adapt_jump=adapt)
+    W = shaped_weights(c, lp)
+    B, seed = 2, 31
+    k = np.random.default_rng(2).choice([1, 2, 3, 6], c.n)
+    om = oracle_fly(c, None, lp, None, B, seed, None, with_optic=False)
+    om.flags = 32 | (8 if adapt > 0 else 0)
+    om.st.poisson_p[:, inputs] = (hz * np.float32(lp.dt / 1000.0)).astype(np.float32)
+    om.set_clocks(k, W, lp)
+    rb = RefClockedBrain(W, B, lp, np.ones(c.n, np.float32))
+    rb.set_clocks(k, W)
+    rb.poisson_p.copy_(torch.from_numpy(om.st.poisson_p))
+    fired = 0
+    for step, st in enumerate(om.lif_steps(240)):
+        rb.step(torch.from_numpy(oracle.uniform(B, c.n, seed, step)))
+        fired += int(st.spikes.sum())
+        for name in ("spikes", "v", "g", "refrac", "rate", "spike_counts", "adapt"):
+            assert torch.equal(getattr(rb, name), torch.from_numpy(getattr(st, name))), (step, name)
+        for kk, acc in om.clocks["acc"].items():
+            assert torch.equal(rb._acc[kk], torch.from_numpy(acc)), (step, kk)
     assert fired > 100

@@ -653,3 +653,83 @@ def test_brain_with_depression_and_adaptation_bit_exact():
         assert_bits(N_(getattr(brain, k)), getattr(om.st, k), k)
     assert_bits(N_(brain.spike_buf), om.st.spike_buf, "spike_buf")
     assert om.st.spike_counts.sum() > 0
+
+
+@pytest.mark.parametrize("B,adapt", [(1, 0.0), (2, 1.5)])
+def test_brain_clocks_bit_exact(B, adapt):
+    """Brain.set_clocks (flyverse/brain.py:618-646) with clocks of 1, 2, 3 and 6 steps: the clocked step on the HIP
+    kernels (launch_clock_acc, launch_pull_clocked, launch_lif_rng with the phase rows, flag 16) against the oracle's
+    clocked composite, every step: spikes, state and the clock accumulators."""
+    c, inputs, hz = synth.plain_lif_graph(7, n=3000, n_inputs=150, hz=400.0)
+    lp = LIFParams.shiu(dt=0.1, adapt_jump=adapt)
+    k = np.random.default_rng(3).choice([1, 2, 3, 6], c.n)
+    brain = Brain(c, lp, device=DEV, batch=B, seed=77)
+    brain.record_activity = True
+    brain.set_poisson(inputs, hz)
+    brain.set_clocks(k)
+    assert brain.K == 6 and sorted(brain._acc) == [2, 3, 6]
+    om = oracle_fly(c, None, lp, None, B, 77, None, with_optic=False)
+    om.flags = 32 | (8 if adapt > 0 else 0)
+    om.st.poisson_p[:, inputs] = (hz * np.float32(lp.dt / 1000.0)).astype(np.float32)
+    om.set_clocks(k, om.W, lp)
+    total = 0
+    for step, st in enumerate(om.lif_steps(150)):
+        brain.step()
+        assert_bits(N(brain.spikes), st.spikes, f"spikes step {step}")
+        total += int(st.spikes.sum())
+        if step % 10 == 9:
+            for name in ("v", "g", "refrac", "rate", "spike_counts", "adapt"):
+                assert_bits(N(getattr(brain, name)), getattr(st, name), f"{name} step {step}")
+            for kk, acc in om.clocks["acc"].items():
+                assert_bits(N(brain._acc[kk]), acc, f"accumulator {kk} step {step}")
+    assert total > 100
+    with pytest.raises(ValueError, match="slowest clock"):
+        brain.set_clocks(np.full(c.n, 40))
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_fly_dt_by_module_bit_exact(graphs):
+    """FlyBrain with LIFParams.dt_by_module (flyverse/fly.py:98-107): the central cells on a 0.2 ms clock, the rest
+    on 0.1 ms; frames with the optic lobe against the oracle's clocked composite, captured frames included."""
+    from dataclasses import replace
+    from flyhip.connectome import clock_multipliers
+    conn, retina, layout, lp, op = fly_setup(0.02, seed=3)
+    lp = replace(lp, dt_by_module={"central": 0.2})
+    k = clock_multipliers(conn, lp.dt_by_module, lp.dt)
+    assert set(np.unique(k)) == {1, 2}
+    idx, hz = synth.fly_inputs(8, layout, 40)
+    fb = FlyBrain(conn, retina, batch=2, device=DEV, seed=5, lif_params=lp, optic_params=op, cuda_graphs=graphs)
+    fb.set_poisson(idx, hz)
+    fb.random_movie(9)
+    assert fb.brain.K == 2
+    om = oracle_fly(conn, retina, lp, op, 2, 5, 9)
+    om.st.poisson_p[:, idx] = (hz * np.float32(lp.dt / 1000.0)).astype(np.float32)
+    om.set_clocks(k, om.W, lp)
+    for frame in range(4):
+        raster = []
+        om.frame(100, raster)
+        fb.step(10.0)
+        torch.cuda.synchronize()
+        assert_bits(N(fb.brain.spikes), raster[-1], f"spikes frame {frame}")
+        for name in ("v", "g", "rate", "spike_counts"):
+            assert_bits(N(getattr(fb.brain, name)), getattr(om.st, name), f"{name} frame {frame}")
+        assert_bits(N(fb.optic.v), om.os.v, f"optic v frame {frame}")
+    assert om.st.spike_counts.sum() > 0
+    # a checkpoint carries the clocks and their accumulators: a clocked twin continues with the same bits, an
+    # unclocked brain refuses it (flyverse/fly.py:762-768)
+    state = fb.state_dict()
+    twin = FlyBrain(conn, retina, batch=2, device=DEV, seed=5, lif_params=lp, optic_params=op, cuda_graphs=graphs)
+    twin.set_poisson(idx, hz)
+    twin.load_state_dict(state)
+    fb.step(10.0)
+    twin.step(10.0)
+    for name in ("v", "g", "spike_counts"):
+        assert torch.equal(getattr(fb.brain, name), getattr(twin.brain, name)), name
+    plain = FlyBrain(conn, retina, batch=2, device=DEV, seed=5, lif_params=replace(lp, dt_by_module=None),
+                     optic_params=op)
+    with pytest.raises(ValueError, match="do not match"):
+        plain.load_state_dict(state)
+    with pytest.raises(ValueError, match="unknown module"):
+        clock_multipliers(conn, {"nowhere": 0.2}, lp.dt)
+    with pytest.raises(ValueError, match="positive multiple"):
+        clock_multipliers(conn, {"central": 0.15}, lp.dt)
