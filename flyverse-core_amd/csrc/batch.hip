@@ -18,6 +18,7 @@
 // one add and one select for all 64 flies, and lanes whose bit is clear are untouched, exactly like the reference
 // skipping silent presynaptic cells.
 #include "common.h"
+#include "tickets.h"
 
 namespace {
 
@@ -545,8 +546,8 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
 // ------------------------------------------------------------------ optic lobe, fly-minor
 // Chunked products over a (cols, 64) input with chunk-padded storage: row r owns chunks [chunk_ptr[r],
 // chunk_ptr[r+1]) (at least one: an empty row has one all-pad chunk), chunk c is entries [16 c, 16 c + 16) of
-// pidx / pw, pads have idx -1 and are skipped.  A wavefront walks the chunks of its rows (sched order, stride =
-// number of wavefronts) as one stream, three chunks deep: the indices and weights of chunk c + 2 arrive by one vector
+// pidx / pw, pads have idx -1 and are skipped.  Wavefront `wave` of `nw` walks the chunks of its rows (sched order
+// from `wave`, stride nw) as one stream, three chunks deep: the indices and weights of chunk c + 2 arrive by one vector
 // load (lanes 0-15 the indices, lanes 16-31 the weights), the 16 row gathers of chunk c + 1 (256 bytes each) are in
 // flight, and chunk c is folded.  Vector loads complete in issue order, so each wait leaves the younger stages' loads
 // outstanding.  The row after the leading cursor's is fetched (scalar loads) when the cursor enters a row, so moving
@@ -557,8 +558,8 @@ struct FmCursor { int si, r, c, ce; };
 template <class Finish>
 __device__ __forceinline__ void fm_chunk_stream(const int* __restrict__ chunk_ptr, const int* __restrict__ pidx,
                                                 const float* __restrict__ pw, const int* __restrict__ sched,
-                                                const float* __restrict__ x, int rows, int lane, Finish finish) {
-    const int nw = gridDim.x * 4;
+                                                const float* __restrict__ x, int rows, int lane, int wave, int nw,
+                                                Finish finish) {
     auto row_at = [&](int si) {
         FmCursor q;
         q.si = si;
@@ -567,7 +568,7 @@ __device__ __forceinline__ void fm_chunk_stream(const int* __restrict__ chunk_pt
         q.ce = chunk_ptr[q.r + 1];
         return q;
     };
-    FmCursor q0 = row_at(blockIdx.x * 4 + wave_id());
+    FmCursor q0 = row_at(wave);
     if (q0.si >= rows) return;
     FmCursor nr = row_at(q0.si + nw);                  // the row after the leading cursor's
     // the leading cursor's next chunk (past the last row it stays past the end: si >= rows)
@@ -649,7 +650,8 @@ __global__ __launch_bounds__(256) void k_spmm_fm(const int* __restrict__ chunk_p
                                                  const float* __restrict__ x, float* __restrict__ y, int rows,
                                                  float scale) {
     const int lane = threadIdx.x & 63;
-    fm_chunk_stream(chunk_ptr, pidx, pw, sched, x, rows, lane, [&](int r, float total) {
+    fm_chunk_stream(chunk_ptr, pidx, pw, sched, x, rows, lane, blockIdx.x * 4 + wave_id(), gridDim.x * 4,
+                    [&](int r, float total) {
         if (scale != 1.0f) total = total * scale;
         y[(int64_t)r * 64 + lane] = total;
     });
@@ -691,7 +693,8 @@ __global__ __launch_bounds__(256) void k_optic_substep_fm(const int* __restrict_
                                                           float ga, float aad) {
     const int lane = threadIdx.x & 63;
     const bool live = (*live_mask >> lane) & 1ull;
-    fm_chunk_stream(chunk_ptr, pidx, pw, sched, dr_cur, rows, lane, [&](int r, float total) {
+    fm_chunk_stream(chunk_ptr, pidx, pw, sched, dr_cur, rows, lane, blockIdx.x * 4 + wave_id(), gridDim.x * 4,
+                    [&](int r, float total) {
         optic_update_one(r, (int64_t)r * 64 + lane, total, live, dr_cur, dr_next, v, adapt, pr, spk, a, b, gr, ga,
                          aad);
     });
@@ -710,66 +713,81 @@ __global__ __launch_bounds__(256) void k_optic_dr_fm(const float* __restrict__ v
 // rad[col, c] of lane b = Philox(movie_seed[b], frame[b], col*4 + c, 0, stream 1), the uniform the plain path's
 // launch_uniform gives fly 0 of a one-fly batch; the averaging, state update and contrast are those of
 // csrc/optic.hip's k_photoreceptor.  I_lp, I_mean (5 n_col, 64); a_pr (n_pr, 64).
-__global__ __launch_bounds__(256) void k_photoreceptor_fm(const float* __restrict__ sens, const int* __restrict__ pr_col,
-                                                          const int* __restrict__ cell_ptr,
-                                                          const int* __restrict__ cell_pr,
-                                                          const float* __restrict__ cell_w, float* __restrict__ I_lp,
-                                                          float* __restrict__ I_mean, float* __restrict__ a_pr,
-                                                          const uint64_t* __restrict__ movie_seed,
-                                                          const uint32_t* __restrict__ frame,
-                                                          const uint64_t* __restrict__ fresh_mask,
-                                                          const uint64_t* __restrict__ live_mask,
-                                                          const float* __restrict__ cf, int n_cells) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n_cells * 64) return;
-    const int lane = (int)(t & 63), k = (int)(t >> 6);
-    const int e0 = cell_ptr[k], e1 = cell_ptr[k + 1];
+struct PrFm {
+    const float* sens;
+    const int *pr_col, *cell_ptr, *cell_pr;
+    const float* cell_w;
+    float *I_lp, *I_mean, *a_pr;
+    const uint64_t* movie_seed;
+    const uint32_t* frame;
+    const uint64_t *fresh_mask, *live_mask;
+    const float* cf;
+};
+
+__device__ __forceinline__ void pr_cell_fm(const PrFm& a, int k, int lane) {
+    const int64_t t = (int64_t)k * 64 + lane;
+    const int e0 = a.cell_ptr[k], e1 = a.cell_ptr[k + 1];
     // a cell without photoreceptors feeds nothing: its state stays at the zeros it starts from (its input is 0)
-    if (e0 == e1 || !((*live_mask >> lane) & 1ull)) return;
-    const uint64_t ms = movie_seed[lane];
-    const uint32_t fr = frame[lane];
+    if (e0 == e1 || !((*a.live_mask >> lane) & 1ull)) return;
+    const uint64_t ms = a.movie_seed[lane];
+    const uint32_t fr = a.frame[lane];
     float I = 0.0f;
     for (int cs = e0; cs < e1; cs += 16) {
         const int ce = min(cs + 16, e1);
         float s = 0.0f;
         for (int e = cs; e < ce; ++e) {
-            const int p = cell_pr[e];
-            const uint32_t base = (uint32_t)pr_col[p] * 4u;
+            const int p = a.cell_pr[e];
+            const uint32_t base = (uint32_t)a.pr_col[p] * 4u;
             float Ip = 0.0f;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 float rad = fh_uniform(ms, fr, base + (uint32_t)c, 0u, 1u);
-                float q = rad * sens[p * 4 + c];
+                float q = rad * a.sens[p * 4 + c];
                 Ip = Ip + q;
             }
-            float q = cell_w[e] * Ip;
+            float q = a.cell_w[e] * Ip;
             s = s + q;
         }
         I = I + s;
     }
-    const bool fr0 = (*fresh_mask >> lane) & 1ull;
-    float lp = fr0 ? I : I_lp[t];
-    float mean = fr0 ? I : I_mean[t];
-    float l1 = cf[0] * lp, l2 = cf[1] * I;
+    const bool fr0 = (*a.fresh_mask >> lane) & 1ull;
+    float lp = fr0 ? I : a.I_lp[t];
+    float mean = fr0 ? I : a.I_mean[t];
+    float l1 = a.cf[0] * lp, l2 = a.cf[1] * I;
     lp = l1 + l2;
-    float m1 = cf[2] * mean, m2 = cf[3] * lp;
+    float m1 = a.cf[2] * mean, m2 = a.cf[3] * lp;
     mean = m1 + m2;
-    I_lp[t] = lp; I_mean[t] = mean;
-    float num = lp - mean, den = mean + cf[4];
+    a.I_lp[t] = lp; a.I_mean[t] = mean;
+    float num = lp - mean, den = mean + a.cf[4];
     float c = num / den;
     c = c < -1.0f ? -1.0f : c;
-    c = c > cf[5] ? cf[5] : c;
-    for (int e = e0; e < e1; ++e) a_pr[(int64_t)cell_pr[e] * 64 + lane] = c;
+    c = c > a.cf[5] ? a.cf[5] : c;
+    for (int e = e0; e < e1; ++e) a.a_pr[(int64_t)a.cell_pr[e] * 64 + lane] = c;
+}
+
+__global__ __launch_bounds__(256) void k_photoreceptor_fm(PrFm a, int n_cells) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_cells * 64) return;
+    pr_cell_fm(a, (int)(t >> 6), (int)(t & 63));
+}
+
+__device__ __forceinline__ float fb_value_fm(float rate) {
+    float q = rate / 100.0f;
+    q = q < 0.0f ? 0.0f : q;
+    return q > 3.0f ? 3.0f : q;
+}
+
+__device__ __forceinline__ float drive_value_fm(float y, float gain_out, float clip) {
+    float d = gain_out * y;
+    d = d < -clip ? -clip : d;
+    return d > clip ? clip : d;
 }
 
 __global__ __launch_bounds__(256) void k_feedback_fm(const float* __restrict__ rate, const int* __restrict__ spk_row,
                                                      float* __restrict__ s, int64_t total) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    float q = rate[(int64_t)spk_row[t >> 6] * 64 + (t & 63)] / 100.0f;
-    q = q < 0.0f ? 0.0f : q;
-    q = q > 3.0f ? 3.0f : q;
-    s[t] = q;
+    s[t] = fb_value_fm(rate[(int64_t)spk_row[t >> 6] * 64 + (t & 63)]);
 }
 
 __global__ __launch_bounds__(256) void k_drive_out_fm(const float* __restrict__ y, const int* __restrict__ spk_row,
@@ -777,10 +795,79 @@ __global__ __launch_bounds__(256) void k_drive_out_fm(const float* __restrict__ 
                                                       int64_t total) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    float d = gain_out * y[t];
-    d = d < -clip ? -clip : d;
-    d = d > clip ? clip : d;
-    drive[(int64_t)spk_row[t >> 6] * 64 + (t & 63)] = d;
+    drive[(int64_t)spk_row[t >> 6] * 64 + (t & 63)] = drive_value_fm(y[t], gain_out, clip);
+}
+
+// The frame's optic inputs, fused (flyverse/optic.py:496-517, 572, 579-580): one launch in two phases (work
+// tickets).  Phase 1, workgroups of 4 rows x 64 lanes: the photoreceptor cells, the feedback s of every spiking
+// target and the first substep's dr = clip01(v + b) - b.  Phase 2, only in a frame with substeps: pr_in = gain_in *
+// W_rp a_pr and spk_in = gain_fb * W_rs s, each streamed by `j_prod` workgroups over a_pr and s as phase 1 left them.
+// Values and order are those of the separate kernels.
+struct FrameInFm {
+    PrFm pr;
+    const float* rate;     // (rows, 64) LIF rates
+    const int* spk_row;    // (n_spk)
+    float* s;              // (n_spk, 64)
+    const float *v, *b;    // (n_rate, 64), (n_rate)
+    float* dr;             // (n_rate, 64)
+    const int *rp_ptr, *rp_idx, *rp_sched;
+    const float* rp_w;
+    const int *rs_ptr, *rs_idx, *rs_sched;
+    const float* rs_w;
+    float *pr_in, *spk_in; // (n_rate, 64)
+    float gain_in, gain_fb;
+    uint32_t* tickets;
+    int n_cells, n_spk, n_rate, j_cells, j_spk, j_rate, j_prod;
+};
+
+__global__ __launch_bounds__(256) void k_frame_in_fm(FrameInFm a) {
+    const uint32_t t = fh_tickets::take(a.tickets);
+    const uint32_t n1 = (uint32_t)(a.j_cells + a.j_spk + a.j_rate), total = n1 + 2u * (uint32_t)a.j_prod;
+    const int lane = threadIdx.x & 63;
+    if (t < n1) {
+        int j = (int)t;
+        const int row = (int)(threadIdx.x >> 6);
+        if (j < a.j_cells) {
+            const int k = j * 4 + row;
+            if (k < a.n_cells) pr_cell_fm(a.pr, k, lane);
+        } else if ((j -= a.j_cells) < a.j_spk) {
+            const int k = j * 4 + row;
+            if (k < a.n_spk) a.s[(int64_t)k * 64 + lane] = fb_value_fm(a.rate[(int64_t)a.spk_row[k] * 64 + lane]);
+        } else {
+            const int r = (j - a.j_spk) * 4 + row;
+            if (r < a.n_rate) {
+                const int64_t q = (int64_t)r * 64 + lane;
+                const float base = a.b[r];
+                const float x = a.v[q] + base;
+                a.dr[q] = fh_clip01(x) - base;
+            }
+        }
+        fh_tickets::publish(a.tickets);
+    } else {
+        const int u = (int)(t - n1);
+        fh_tickets::wait(a.tickets, n1);
+        const int nw = a.j_prod * 4;
+        if (u < a.j_prod)
+            fm_chunk_stream(a.rp_ptr, a.rp_idx, a.rp_w, a.rp_sched, a.pr.a_pr, a.n_rate, lane, u * 4 + wave_id(), nw,
+                            [&](int r, float tot) { a.pr_in[(int64_t)r * 64 + lane] = tot * a.gain_in; });
+        else
+            fm_chunk_stream(a.rs_ptr, a.rs_idx, a.rs_w, a.rs_sched, a.s, a.n_rate, lane, (u - a.j_prod) * 4 + wave_id(),
+                            nw, [&](int r, float tot) { a.spk_in[(int64_t)r * 64 + lane] = tot * a.gain_fb; });
+    }
+    fh_tickets::leave(a.tickets, total);
+}
+
+// The frame's optic output into the LIF drive, fused (flyverse/optic.py:599-604): drive[spk_row[k], b] =
+// clamp(gain_out * (W_sr dr)[k, b]) for dr (n_rate, 64) = clip01(v + b) - b (the last substep's output).
+__global__ __launch_bounds__(256) void k_frame_out_fm(const int* __restrict__ chunk_ptr, const int* __restrict__ pidx,
+                                                      const float* __restrict__ pw, const int* __restrict__ sched,
+                                                      const float* __restrict__ dr, const int* __restrict__ spk_row,
+                                                      float* __restrict__ drive, int rows, float gain_out, float clip) {
+    const int lane = threadIdx.x & 63;
+    fm_chunk_stream(chunk_ptr, pidx, pw, sched, dr, rows, lane, blockIdx.x * 4 + wave_id(), gridDim.x * 4,
+                    [&](int r, float tot) {
+                        drive[(int64_t)spk_row[r] * 64 + lane] = drive_value_fm(tot, gain_out, clip);
+                    });
 }
 
 // End of a frame: the engine clock and every live fly advance by `steps`; live and fresh masks are refreshed.
@@ -830,6 +917,13 @@ __global__ __launch_bounds__(256) void k_reset_lanes(float* v, float* g, float* 
 }
 
 inline unsigned nblk(int64_t total) { return (unsigned)((total + 255) / 256); }
+
+// The photoreceptor pointers p[0..12] of launch_photoreceptor_fm.
+inline PrFm pr_fm(void** p) {
+    return PrFm{(const float*)p[0], (const int*)p[1], (const int*)p[2], (const int*)p[3], (const float*)p[4],
+                (float*)p[5], (float*)p[6], (float*)p[7], (const uint64_t*)p[8], (const uint32_t*)p[9],
+                (const uint64_t*)p[10], (const uint64_t*)p[11], (const float*)p[12]};
+}
 
 // The activity bits of one k_batch_lif<6> launch (slot-major ring): act[r] bit k is set when the mask of row r in
 // the slot that step k of the launch reads, (T0 + k - delay) mod ring_slots, has any fly's bit set.  Those slots
@@ -940,9 +1034,7 @@ FH_EXPORT int launch_photoreceptor_fm(void** p, int n_cells, void* stream) {
     if (n_cells < 0) return FH_ERR_ARGS;
     if (!n_cells) return 0;
     hipLaunchKernelGGL(k_photoreceptor_fm, dim3(nblk((int64_t)n_cells * 64)), dim3(256), 0, (hipStream_t)stream,
-                       (const float*)p[0], (const int*)p[1], (const int*)p[2], (const int*)p[3], (const float*)p[4],
-                       (float*)p[5], (float*)p[6], (float*)p[7], (const uint64_t*)p[8], (const uint32_t*)p[9],
-                       (const uint64_t*)p[10], (const uint64_t*)p[11], (const float*)p[12], n_cells);
+                       pr_fm(p), n_cells);
     FH_LAUNCH_CHECK();
 }
 
@@ -963,6 +1055,45 @@ FH_EXPORT int launch_drive_out_fm(void** p, int n_spk, float gain_out, float cli
     int64_t total = (int64_t)n_spk * 64;
     hipLaunchKernelGGL(k_drive_out_fm, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, (const float*)p[0],
                        (const int*)p[1], (float*)p[2], gain_out, clip, total);
+    FH_LAUNCH_CHECK();
+}
+
+// The frame's optic inputs in one launch: p = [the 13 pointers of launch_photoreceptor_fm, rate (rows, 64),
+// spk_row (n_spk), s (n_spk, 64), v (n_rate, 64), b (n_rate), dr (n_rate, 64), W_rp chunk_ptr, pidx, pw, sched,
+// W_rs chunk_ptr, pidx, pw, sched, pr_in (n_rate, 64), spk_in (n_rate, 64), tickets uint32 (4, zero before the first
+// launch)]; dims = [n_cells, n_spk, n_rate, grid]: grid workgroups stream each product, 0 in a frame without substeps.
+// Launches that share `tickets` must not overlap.
+FH_EXPORT int launch_frame_in_fm(void** p, const int* dims, float gain_in, float gain_fb, void* stream) {
+    for (int i = 0; i < 4; ++i)
+        if (dims[i] < 0) return FH_ERR_ARGS;
+    FrameInFm a;
+    a.pr = pr_fm(p);
+    a.rate = (const float*)p[13]; a.spk_row = (const int*)p[14]; a.s = (float*)p[15]; a.v = (const float*)p[16];
+    a.b = (const float*)p[17]; a.dr = (float*)p[18];
+    a.rp_ptr = (const int*)p[19]; a.rp_idx = (const int*)p[20]; a.rp_w = (const float*)p[21];
+    a.rp_sched = (const int*)p[22];
+    a.rs_ptr = (const int*)p[23]; a.rs_idx = (const int*)p[24]; a.rs_w = (const float*)p[25];
+    a.rs_sched = (const int*)p[26];
+    a.pr_in = (float*)p[27]; a.spk_in = (float*)p[28]; a.tickets = (uint32_t*)p[29];
+    a.gain_in = gain_in; a.gain_fb = gain_fb;
+    a.n_cells = dims[0]; a.n_spk = dims[1]; a.n_rate = dims[2];
+    a.j_cells = (a.n_cells + 3) / 4; a.j_spk = (a.n_spk + 3) / 4; a.j_rate = (a.n_rate + 3) / 4;
+    a.j_prod = a.n_rate ? dims[3] : 0;
+    const int64_t grid = (int64_t)a.j_cells + a.j_spk + a.j_rate + 2ll * a.j_prod;
+    if (grid >= (1ll << 31)) return FH_ERR_SIZE;
+    if (!grid) return 0;
+    hipLaunchKernelGGL(k_frame_in_fm, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, a);
+    FH_LAUNCH_CHECK();
+}
+
+// The frame's optic output into the LIF drive in one launch: p = [W_sr chunk_ptr, pidx, pw, sched, dr (n_rate, 64),
+// spk_row (n_spk), drive (rows, 64)]; dr must hold clip01(v + b) - b (the last substep's output).
+FH_EXPORT int launch_frame_out_fm(void** p, int n_spk, float gain_out, float clip, int grid, void* stream) {
+    if (n_spk < 0 || grid <= 0) return FH_ERR_ARGS;
+    if (!n_spk) return 0;
+    hipLaunchKernelGGL(k_frame_out_fm, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
+                       (const int*)p[1], (const float*)p[2], (const int*)p[3], (const float*)p[4], (const int*)p[5],
+                       (float*)p[6], n_spk, gain_out, clip);
     FH_LAUNCH_CHECK();
 }
 

@@ -5,6 +5,7 @@
 // product rounded first), and the row total is a sequential fold of the chunk sums from +0.  The reference's CPU
 // order (torch sparse mm) is not pinned by anything it ships; agreement with it is to 1e-5 relative.
 #include "common.h"
+#include "groups.h"
 
 #include <mutex>
 
@@ -93,61 +94,13 @@ __global__ __launch_bounds__(256) void k_spmv_groups(const int* __restrict__ gro
     if (g >= ngroups) return;
     const int b = blockIdx.y;
     const float* xb = x + (int64_t)b * cols;
-    const int r0 = group_row[g], r1 = group_row[g + 1];
-    const int c0 = chunk_ptr[r0], c1 = chunk_ptr[r1];
-    auto chunk_sum = [&](int c) {
-        const int4* ip = pidx + (int64_t)c * 4;
-        const float4* wp = pw + (int64_t)c * 4;
-        int4 iv[4];
-        float4 wv4[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { iv[q] = ip[q]; wv4[q] = wp[q]; }
-        float xv[16];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            xv[4 * q + 0] = xb[max(iv[q].x, 0)];
-            xv[4 * q + 1] = xb[max(iv[q].y, 0)];
-            xv[4 * q + 2] = xb[max(iv[q].z, 0)];
-            xv[4 * q + 3] = xb[max(iv[q].w, 0)];
-        }
-        float s = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float c_;
-            c_ = wv4[q].x * xv[4 * q + 0]; if (iv[q].x >= 0) s = s + c_;
-            c_ = wv4[q].y * xv[4 * q + 1]; if (iv[q].y >= 0) s = s + c_;
-            c_ = wv4[q].z * xv[4 * q + 2]; if (iv[q].z >= 0) s = s + c_;
-            c_ = wv4[q].w * xv[4 * q + 3]; if (iv[q].w >= 0) s = s + c_;
-        }
-        return s;
-    };
-    auto finish = [&](float total, int row) {
+    const fh_groups::Grouped m{group_row, chunk_ptr, pidx, pw, ngroups};
+    fh_groups::group_product(m, g, lane, sums[wv], [&](int i) { return xb[i]; }, [&](int row, float total) {
         if (scale != 1.0f) total = total * scale;
         float* yp = y + (int64_t)b * rows + row;
         if (accumulate) total = *yp + total;
         *yp = total;
-    };
-    if (c1 - c0 <= kWave) {
-        const float s = lane < c1 - c0 ? chunk_sum(c0 + lane) : 0.0f;
-        sums[wv][lane] = s;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane < r1 - r0) {
-            const int row = r0 + lane;
-            float total = 0.0f;
-            for (int c = chunk_ptr[row] - c0; c < chunk_ptr[row + 1] - c0; ++c) total = total + sums[wv][c];
-            finish(total, row);
-        }
-    } else {                                       // one long row
-        float total = 0.0f;
-        for (int cb = c0; cb < c1; cb += kWave) {
-            const float s = cb + lane < c1 ? chunk_sum(cb + lane) : 0.0f;
-            const int n = min(kWave, c1 - cb);
-            for (int j = 0; j < n; ++j) total = total + fh_readlane(s, j);
-        }
-        if (lane == 0) finish(total, r0);
-    }
+    });
 }
 
 // LDS-staged product over chunk-padded storage: the gathers of x go to LDS instead of the vector caches (a 4-byte
