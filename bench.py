@@ -6,8 +6,9 @@ Shiu LIF at dt 0.1 ms coupled to the 89,390-unit optic lobe every 10 ms frame (1
     photoreceptor movie seed, 300 Poisson inputs at U[0, 200] Hz and a length uniform over 2,000-10,000 LIF steps
     (whole 10 ms frames); finished lanes are refilled.  One bench step = one 10 ms frame of all 64 lanes, replayed
     from a captured HIP graph.  value = simulated fly-seconds / wall-seconds over the timed frames.
-  * B = 1: flyhip.FlyBrain(batch=1, cuda_graphs=True), the plain kernels (launch_pull_bits + launch_lif_bits per step),
-    one captured graph per frame shape; reported as b1_sim_s_per_wall_s and the ratio.
+  * B = 1: flyhip.FlyBrain(batch=1, cuda_graphs=True), the plain kernels (window launches of up to 18 LIF steps,
+    launch_lif_window; the fused frame-start / frame-end optic launches around 10 substeps), one captured graph per
+    frame shape; reported as b1_sim_s_per_wall_s and the ratio.
   * roofline: the batched LIF launch (csrc/batch.hip k_batch_lif, <= n_delay steps per launch): one frame of its
     launches timed with HIP events on the launch stream right after the timed region; algorithmic bytes per launch
     from the per-step LIF and synaptic-input byte counts (lif_block_bytes).
@@ -242,6 +243,32 @@ def packed_run(steps, log):
     return out
 
 
+def b1_run(conn, retina, lp, op, r, frames):
+    """The plain single-fly path: FlyBrain(batch=1, cuda_graphs=True) with rollout r's inputs and movie, 10 frames to
+    capture every delay-ring phase of a 100-step frame, then `frames` timed 10 ms frames."""
+    import torch
+    from flyhip.fly import FlyBrain
+    log("B=1 plain path: setup")
+    fb = FlyBrain(conn, retina, batch=1, device="cuda", seed=r.seed, lif_params=lp, optic_params=op, cuda_graphs=True)
+    fb.set_poisson(r.input_idx, r.input_hz)
+    fb.random_movie(r.movie_seed)
+    for _ in range(10):
+        fb.step(10.0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(frames):
+        fb.step(10.0)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    b1 = frames * 0.010 / (ms / 1000.0)
+    log(f"B=1: {frames} frames in {ms:.1f} ms: {b1:.3f} sim-s/wall-s")
+    del fb
+    torch.cuda.empty_cache()
+    return {"b1_sim_s_per_wall_s": round(b1, 4), "b1_ms_per_frame": round(ms / frames, 4)}
+
+
 def copy_calibration(dev):
     """One device-to-device copy of 1 GiB: the kernel of known traffic the PMC byte counters are scaled by."""
     import torch
@@ -320,11 +347,15 @@ def main():
                     help="pin the B=64 launch shape (fold mode, steps per launch, workgroups per CU); skips tuning")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-optic-training", action="store_true")
-    ap.add_argument("--only-b64", action="store_true",
-                    help="only the B=64 engine (plus --copy-calibration): the short run the PMC counter passes profile")
+    ap.add_argument("--only", choices=("b64", "spmv", "b1", "optic-training", "packed"), default=None,
+                    help="run one part alone (plus --copy-calibration) and print its figures: the short runs the "
+                         "kernel traces and PMC counter passes profile")
+    ap.add_argument("--only-b64", action="store_true", help="the same as --only b64")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
+    if args.only_b64:
+        args.only = "b64"
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this workload shards by fly with no communication, and runs on more than "
               f"one GPU are not part of this build (asked for --gpus {args.gpus})", file=sys.stderr)
@@ -334,7 +365,6 @@ def main():
     from flyhip import synth
     from flyhip.batch import BatchBrain
     from flyhip.brain import LIFParams
-    from flyhip.fly import FlyBrain
     from flyhip.optic import OpticParams
 
     dev = torch.device("cuda:0")
@@ -344,6 +374,15 @@ def main():
     lp = LIFParams.shiu(dt=0.1)
     op = OpticParams(tau_by_type=conn.rate_tau, baseline_by_type=conn.rate_baseline, pair_gain=[])
     rollouts = make_rollouts(layout, 64 + 4 * 64, seed=11)
+
+    if args.only not in (None, "b64"):
+        part = {"spmv": lambda: spmv_roofline(conn, retina, op), "b1": lambda: b1_run(conn, retina, lp, op, rollouts[0],
+                                                                                        args.b1_frames),
+                "optic-training": optic_training, "packed": lambda: packed_run(args.packed_steps, log)}[args.only]()
+        if args.copy_calibration:
+            copy_calibration(dev)
+        print(json.dumps(part), flush=True)
+        return
 
     # ---------------------------------------------------------------- B = 64, batched engine
     log("B=64 batched engine: setup")
@@ -386,7 +425,7 @@ def main():
     traffic, traffic_shape = measured_traffic(tuned)
     del bb
     torch.cuda.empty_cache()
-    if args.only_b64:
+    if args.only == "b64":
         if args.copy_calibration:
             copy_calibration(dev)
         print(json.dumps({"b64_sim_s_per_wall_s": round(value, 4), "lif_launch_ms": round(lif_ms, 4)}), flush=True)
@@ -398,24 +437,8 @@ def main():
         f"{ {k: v for k, v in spmv.items() if k.endswith('_us') and k != 'spmv_wrr_b1_us'} }")
 
     # ---------------------------------------------------------------- B = 1, plain kernels
-    log("B=1 plain path: setup")
-    r = rollouts[0]
-    fb = FlyBrain(conn, retina, batch=1, device=dev, seed=r.seed, lif_params=lp, optic_params=op, cuda_graphs=True)
-    fb.set_poisson(r.input_idx, r.input_hz)
-    fb.random_movie(r.movie_seed)
-    for _ in range(10):                      # every delay-ring phase of a 100-step frame captured once
-        fb.step(10.0)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(args.b1_frames):
-        fb.step(10.0)
-    e1.record()
-    torch.cuda.synchronize()
-    b1_ms = e0.elapsed_time(e1)
-    b1 = args.b1_frames * 0.010 / (b1_ms / 1000.0)
-    log(f"B=1: {args.b1_frames} frames in {b1_ms:.1f} ms: {b1:.3f} sim-s/wall-s")
-    del fb
-    torch.cuda.empty_cache()
+    b1_part = b1_run(conn, retina, lp, op, rollouts[0], args.b1_frames)
+    b1, b1_ms_frame = b1_part["b1_sim_s_per_wall_s"], b1_part["b1_ms_per_frame"]
 
     ol = None
     if not args.no_optic_training:
@@ -459,8 +482,8 @@ def main():
                                "B=64 batched, rollouts uniform 2,000-10,000 steps with lane refill; one step = one "
                                "10 ms frame",
                    "batch": 64, "neurons": int(conn.n), "edges": int(conn.W.nnz), "dt_ms": lp.dt,
-                   "b1_sim_s_per_wall_s": round(b1, 4), "ratio_b64_vs_b1": round(value / b1, 2),
-                   "b1_ms_per_frame": round(b1_ms / args.b1_frames, 4), "spike_activity_per_step": activity,
+                   "b1_sim_s_per_wall_s": b1, "ratio_b64_vs_b1": round(value / b1, 2),
+                   "b1_ms_per_frame": b1_ms_frame, "spike_activity_per_step": activity,
                    "b64_launch_shape": tuned, **spmv, **(ol or {}), **(packed or {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

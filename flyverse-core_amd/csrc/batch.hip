@@ -18,7 +18,6 @@
 // one add and one select for all 64 flies, and lanes whose bit is clear are untouched, exactly like the reference
 // skipping silent presynaptic cells.
 #include "common.h"
-#include "tickets.h"
 
 namespace {
 
@@ -798,11 +797,11 @@ __global__ __launch_bounds__(256) void k_drive_out_fm(const float* __restrict__ 
     drive[(int64_t)spk_row[t >> 6] * 64 + (t & 63)] = drive_value_fm(y[t], gain_out, clip);
 }
 
-// The frame's optic inputs, fused (flyverse/optic.py:496-517, 572, 579-580): one launch in two phases (work
-// tickets).  Phase 1, workgroups of 4 rows x 64 lanes: the photoreceptor cells, the feedback s of every spiking
-// target and the first substep's dr = clip01(v + b) - b.  Phase 2, only in a frame with substeps: pr_in = gain_in *
-// W_rp a_pr and spk_in = gain_fb * W_rs s, each streamed by `j_prod` workgroups over a_pr and s as phase 1 left them.
-// Values and order are those of the separate kernels.
+// The frame's optic inputs (flyverse/optic.py:496-517, 572, 579-580) in two launches.  k_frame_cells_fm, workgroups of
+// 4 rows x 64 lanes: the photoreceptor cells, the feedback s of every spiking target and the first substep's
+// dr = clip01(v + b) - b.  k_frame_held_fm, only in a frame with substeps: pr_in = gain_in * W_rp a_pr and
+// spk_in = gain_fb * W_rs s, each streamed by `j_prod` workgroups.  Values and order are those of the separate
+// kernels.
 struct FrameInFm {
     PrFm pr;
     const float* rate;     // (rows, 64) LIF rates
@@ -816,45 +815,38 @@ struct FrameInFm {
     const float* rs_w;
     float *pr_in, *spk_in; // (n_rate, 64)
     float gain_in, gain_fb;
-    uint32_t* tickets;
-    int n_cells, n_spk, n_rate, j_cells, j_spk, j_rate, j_prod;
+    int n_cells, n_spk, n_rate, j_cells, j_spk, j_prod;
 };
 
-__global__ __launch_bounds__(256) void k_frame_in_fm(FrameInFm a) {
-    const uint32_t t = fh_tickets::take(a.tickets);
-    const uint32_t n1 = (uint32_t)(a.j_cells + a.j_spk + a.j_rate), total = n1 + 2u * (uint32_t)a.j_prod;
-    const int lane = threadIdx.x & 63;
-    if (t < n1) {
-        int j = (int)t;
-        const int row = (int)(threadIdx.x >> 6);
-        if (j < a.j_cells) {
-            const int k = j * 4 + row;
-            if (k < a.n_cells) pr_cell_fm(a.pr, k, lane);
-        } else if ((j -= a.j_cells) < a.j_spk) {
-            const int k = j * 4 + row;
-            if (k < a.n_spk) a.s[(int64_t)k * 64 + lane] = fb_value_fm(a.rate[(int64_t)a.spk_row[k] * 64 + lane]);
-        } else {
-            const int r = (j - a.j_spk) * 4 + row;
-            if (r < a.n_rate) {
-                const int64_t q = (int64_t)r * 64 + lane;
-                const float base = a.b[r];
-                const float x = a.v[q] + base;
-                a.dr[q] = fh_clip01(x) - base;
-            }
-        }
-        fh_tickets::publish(a.tickets);
+__global__ __launch_bounds__(256) void k_frame_cells_fm(FrameInFm a) {
+    const int lane = threadIdx.x & 63, row = (int)(threadIdx.x >> 6);
+    int j = blockIdx.x;
+    if (j < a.j_cells) {
+        const int k = j * 4 + row;
+        if (k < a.n_cells) pr_cell_fm(a.pr, k, lane);
+    } else if ((j -= a.j_cells) < a.j_spk) {
+        const int k = j * 4 + row;
+        if (k < a.n_spk) a.s[(int64_t)k * 64 + lane] = fb_value_fm(a.rate[(int64_t)a.spk_row[k] * 64 + lane]);
     } else {
-        const int u = (int)(t - n1);
-        fh_tickets::wait(a.tickets, n1);
-        const int nw = a.j_prod * 4;
-        if (u < a.j_prod)
-            fm_chunk_stream(a.rp_ptr, a.rp_idx, a.rp_w, a.rp_sched, a.pr.a_pr, a.n_rate, lane, u * 4 + wave_id(), nw,
-                            [&](int r, float tot) { a.pr_in[(int64_t)r * 64 + lane] = tot * a.gain_in; });
-        else
-            fm_chunk_stream(a.rs_ptr, a.rs_idx, a.rs_w, a.rs_sched, a.s, a.n_rate, lane, (u - a.j_prod) * 4 + wave_id(),
-                            nw, [&](int r, float tot) { a.spk_in[(int64_t)r * 64 + lane] = tot * a.gain_fb; });
+        const int r = (j - a.j_spk) * 4 + row;
+        if (r < a.n_rate) {
+            const int64_t q = (int64_t)r * 64 + lane;
+            const float base = a.b[r];
+            const float x = a.v[q] + base;
+            a.dr[q] = fh_clip01(x) - base;
+        }
     }
-    fh_tickets::leave(a.tickets, total);
+}
+
+__global__ __launch_bounds__(256) void k_frame_held_fm(FrameInFm a) {
+    const int lane = threadIdx.x & 63;
+    const int u = blockIdx.x, nw = a.j_prod * 4;
+    if (u < a.j_prod)
+        fm_chunk_stream(a.rp_ptr, a.rp_idx, a.rp_w, a.rp_sched, a.pr.a_pr, a.n_rate, lane, u * 4 + wave_id(), nw,
+                        [&](int r, float tot) { a.pr_in[(int64_t)r * 64 + lane] = tot * a.gain_in; });
+    else
+        fm_chunk_stream(a.rs_ptr, a.rs_idx, a.rs_w, a.rs_sched, a.s, a.n_rate, lane, (u - a.j_prod) * 4 + wave_id(),
+                        nw, [&](int r, float tot) { a.spk_in[(int64_t)r * 64 + lane] = tot * a.gain_fb; });
 }
 
 // The frame's optic output into the LIF drive, fused (flyverse/optic.py:599-604): drive[spk_row[k], b] =
@@ -1058,11 +1050,10 @@ FH_EXPORT int launch_drive_out_fm(void** p, int n_spk, float gain_out, float cli
     FH_LAUNCH_CHECK();
 }
 
-// The frame's optic inputs in one launch: p = [the 13 pointers of launch_photoreceptor_fm, rate (rows, 64),
+// The frame's optic inputs, two launches: p = [the 13 pointers of launch_photoreceptor_fm, rate (rows, 64),
 // spk_row (n_spk), s (n_spk, 64), v (n_rate, 64), b (n_rate), dr (n_rate, 64), W_rp chunk_ptr, pidx, pw, sched,
-// W_rs chunk_ptr, pidx, pw, sched, pr_in (n_rate, 64), spk_in (n_rate, 64), tickets uint32 (4, zero before the first
-// launch)]; dims = [n_cells, n_spk, n_rate, grid]: grid workgroups stream each product, 0 in a frame without substeps.
-// Launches that share `tickets` must not overlap.
+// W_rs chunk_ptr, pidx, pw, sched, pr_in (n_rate, 64), spk_in (n_rate, 64)]; dims = [n_cells, n_spk, n_rate, grid]:
+// grid workgroups stream each product, 0 in a frame without substeps.
 FH_EXPORT int launch_frame_in_fm(void** p, const int* dims, float gain_in, float gain_fb, void* stream) {
     for (int i = 0; i < 4; ++i)
         if (dims[i] < 0) return FH_ERR_ARGS;
@@ -1074,15 +1065,19 @@ FH_EXPORT int launch_frame_in_fm(void** p, const int* dims, float gain_in, float
     a.rp_sched = (const int*)p[22];
     a.rs_ptr = (const int*)p[23]; a.rs_idx = (const int*)p[24]; a.rs_w = (const float*)p[25];
     a.rs_sched = (const int*)p[26];
-    a.pr_in = (float*)p[27]; a.spk_in = (float*)p[28]; a.tickets = (uint32_t*)p[29];
+    a.pr_in = (float*)p[27]; a.spk_in = (float*)p[28];
     a.gain_in = gain_in; a.gain_fb = gain_fb;
     a.n_cells = dims[0]; a.n_spk = dims[1]; a.n_rate = dims[2];
-    a.j_cells = (a.n_cells + 3) / 4; a.j_spk = (a.n_spk + 3) / 4; a.j_rate = (a.n_rate + 3) / 4;
+    a.j_cells = (a.n_cells + 3) / 4; a.j_spk = (a.n_spk + 3) / 4;
     a.j_prod = a.n_rate ? dims[3] : 0;
-    const int64_t grid = (int64_t)a.j_cells + a.j_spk + a.j_rate + 2ll * a.j_prod;
-    if (grid >= (1ll << 31)) return FH_ERR_SIZE;
-    if (!grid) return 0;
-    hipLaunchKernelGGL(k_frame_in_fm, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, a);
+    const int64_t g1 = (int64_t)a.j_cells + a.j_spk + (a.n_rate + 3) / 4;
+    if (g1 >= (1ll << 31) || dims[3] >= (1 << 30)) return FH_ERR_SIZE;
+    if (g1) {
+        hipLaunchKernelGGL(k_frame_cells_fm, dim3((unsigned)g1), dim3(256), 0, (hipStream_t)stream, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
+    }
+    if (a.j_prod) hipLaunchKernelGGL(k_frame_held_fm, dim3(2 * a.j_prod), dim3(256), 0, (hipStream_t)stream, a);
     FH_LAUNCH_CHECK();
 }
 

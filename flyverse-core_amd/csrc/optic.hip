@@ -3,7 +3,6 @@
 // photoreceptor stage optic.py:496-517, the feedback optic.py:572 and the output drive optic.py:599-602.
 #include "common.h"
 #include "groups.h"
-#include "tickets.h"
 
 namespace {
 
@@ -150,15 +149,16 @@ __global__ __launch_bounds__(256) void k_drive_out(const float* __restrict__ y_s
 }
 
 // ------------------------------------------------------------------ the frame's optic inputs and outputs, fused
-// Frame start (flyverse/optic.py:496-517, 572, 579-580): one launch in two phases (work tickets).  Phase 1, per fly:
-// the photoreceptor cells (state, contrast, activity a_pr), the feedback s of every spiking target, and the first
-// substep's dr = clip01(v + b) - b.  Phase 2, per fly, only in a frame with substeps: pr_in = gain_in * W_rp a_pr and
-// spk_in = gain_fb * W_rs s, grouped products over a_pr and s as phase 1 left them.  The last workgroup clears the
-// fresh flags (read by every photoreceptor cell of phase 1).  Values and order are those of the separate kernels.
+// Frame start (flyverse/optic.py:496-517, 572, 579-580) in two launches.  k_frame_cells, per fly: the photoreceptor
+// cells (state, contrast, activity a_pr), the feedback s of every spiking target, and the first substep's
+// dr = clip01(v + b) - b.  k_frame_held, per fly, only in a frame with substeps: pr_in = gain_in * W_rp a_pr and
+// spk_in = gain_fb * W_rs s, grouped products over a_pr and s as the first launch left them.  (Within one launch,
+// ordering the products after every photoreceptor cell takes a device-wide count per workgroup; on this GPU those
+// same-address atomics cost far more than the launch boundary.)  Values and order are those of the separate kernels.
 struct FrameIn {
     PrCells pc;
     float *I_lp, *I_mean, *contrast, *a_pr;
-    uint8_t* fresh;
+    const uint8_t* fresh;
     const float* cf;
     const float* rate;      // (B, n) LIF rates
     const int* spk_idx;     // (n_spk)
@@ -168,74 +168,69 @@ struct FrameIn {
     fh_groups::Grouped rp, rs;
     float *pr_in, *spk_in;  // (B, n_rate)
     float gain_in, gain_fb;
-    uint32_t* tickets;
-    int n_col, n_pr, n, n_spk, n_rate, batch;
-    int j_cells, j_spk, j_rate, j_rp, j_rs;   // workgroups per fly of each job kind
+    int n_col, n_pr, n, n_spk, n_rate;
+    int j_cells, j_spk, j_rate, j_rp;   // workgroups per fly of each job kind
 };
 
-__global__ __launch_bounds__(256) void k_frame_in(FrameIn a) {
-    __shared__ float sums[4][kWave];
-    const uint32_t t = fh_tickets::take(a.tickets);
-    const int per1 = a.j_cells + a.j_spk + a.j_rate, per2 = a.j_rp + a.j_rs;
-    const uint32_t n1 = (uint32_t)per1 * a.batch, total = n1 + (uint32_t)per2 * a.batch;
-    if (t < n1) {
-        const int b = (int)(t / per1);
-        int j = (int)(t % per1);
-        if (j < a.j_cells) {
-            const int k = j * 256 + threadIdx.x;
-            if (k < a.n_col * 5) pr_cell(a.pc, a.I_lp, a.I_mean, a.fresh, a.cf, a.contrast, a.a_pr, a.n_col, a.n_pr, b, k);
-        } else if ((j -= a.j_cells) < a.j_spk) {
-            const int k = j * 256 + threadIdx.x;
-            if (k < a.n_spk) a.s[(int64_t)b * a.n_spk + k] = fb_value(a.rate[(int64_t)b * a.n + a.spk_idx[k]]);
-        } else {
-            const int r = (j - a.j_spk) * 256 + threadIdx.x;
-            if (r < a.n_rate) a.dr[(int64_t)b * a.n_rate + r] = dr_value(a.v[(int64_t)b * a.n_rate + r], a.bvec[r]);
-        }
-        fh_tickets::publish(a.tickets);
+__global__ __launch_bounds__(256) void k_frame_cells(FrameIn a) {
+    const int b = blockIdx.y;
+    int j = blockIdx.x;
+    if (j < a.j_cells) {
+        const int k = j * 256 + threadIdx.x;
+        if (k < a.n_col * 5) pr_cell(a.pc, a.I_lp, a.I_mean, a.fresh, a.cf, a.contrast, a.a_pr, a.n_col, a.n_pr, b, k);
+    } else if ((j -= a.j_cells) < a.j_spk) {
+        const int k = j * 256 + threadIdx.x;
+        if (k < a.n_spk) a.s[(int64_t)b * a.n_spk + k] = fb_value(a.rate[(int64_t)b * a.n + a.spk_idx[k]]);
     } else {
-        const uint32_t u = t - n1;
-        const int b = (int)(u / per2), j = (int)(u % per2);
-        fh_tickets::wait(a.tickets, n1);
-        const int lane = threadIdx.x & 63;
-        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        if (j < a.j_rp) {
-            const int g = j * 4 + wv;
-            const float* x = a.a_pr + (int64_t)b * a.n_pr;
-            float* y = a.pr_in + (int64_t)b * a.n_rate;
-            if (g < a.rp.ngroups)
-                fh_groups::group_product(a.rp, g, lane, sums[wv], [&](int i) { return x[i]; },
-                                         [&](int row, float tot) { y[row] = tot * a.gain_in; });
-        } else {
-            const int g = (j - a.j_rp) * 4 + wv;
-            const float* x = a.s + (int64_t)b * a.n_spk;
-            float* y = a.spk_in + (int64_t)b * a.n_rate;
-            if (g < a.rs.ngroups)
-                fh_groups::group_product(a.rs, g, lane, sums[wv], [&](int i) { return x[i]; },
-                                         [&](int row, float tot) { y[row] = tot * a.gain_fb; });
-        }
+        const int r = (j - a.j_spk) * 256 + threadIdx.x;
+        if (r < a.n_rate) a.dr[(int64_t)b * a.n_rate + r] = dr_value(a.v[(int64_t)b * a.n_rate + r], a.bvec[r]);
     }
-    if (fh_tickets::leave(a.tickets, total))
-        for (int i = threadIdx.x; i < a.batch; i += blockDim.x) a.fresh[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_frame_held(FrameIn a) {
+    __shared__ float sums[4][kWave];
+    const int b = blockIdx.y, j = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (j < a.j_rp) {
+        const int g = j * 4 + wv;
+        const float* x = a.a_pr + (int64_t)b * a.n_pr;
+        float* y = a.pr_in + (int64_t)b * a.n_rate;
+        if (g < a.rp.ngroups)
+            fh_groups::group_product(a.rp, g, lane, sums[wv], [&](int i) { return x[i]; },
+                                     [&](int row, float tot) { y[row] = tot * a.gain_in; });
+    } else {
+        const int g = (j - a.j_rp) * 4 + wv;
+        const float* x = a.s + (int64_t)b * a.n_spk;
+        float* y = a.spk_in + (int64_t)b * a.n_rate;
+        if (g < a.rs.ngroups)
+            fh_groups::group_product(a.rs, g, lane, sums[wv], [&](int i) { return x[i]; },
+                                     [&](int row, float tot) { y[row] = tot * a.gain_fb; });
+    }
 }
 
 // Frame end (flyverse/optic.py:599-604): drive[b, spk_idx[k]] = clamp(gain_out * (W_sr dr)[k]) with
 // dr = clip01(v + b) - b, the rate change from r0 = b.  READY: dr is the last substep's output, read as is (the substep
 // writes exactly this value); otherwise (a frame without substeps) the product forms dr from v as it gathers, and
-// the launch's own rate-unit jobs store it.  No job reads what another writes.
+// the launch's own rate-unit jobs store it.  No job reads what another writes.  Workgroup (0, 0) also clears the
+// fresh flags, which the frame start read.
 struct FrameOut {
     fh_groups::Grouped sr;
     const float *v, *bvec;  // (B, n_rate), (n_rate)
     float* dr;              // (B, n_rate)
     const int* spk_idx;     // (n_spk)
     float* drive;           // (B, n)
+    uint8_t* fresh;         // (B)
     float gain_out, clip;
-    int n, n_spk, n_rate, j_sr;
+    int n, n_spk, n_rate, j_sr, batch;
 };
 
 template <bool READY>
 __global__ __launch_bounds__(256) void k_frame_out(FrameOut a) {
     __shared__ float sums[4][kWave];
     const int b = blockIdx.y, j = blockIdx.x;
+    if (b == 0 && j == 0)
+        for (int i = threadIdx.x; i < a.batch; i += blockDim.x) a.fresh[i] = 0;
     const float* vb = a.v + (int64_t)b * a.n_rate;
     float* drb = a.dr + (int64_t)b * a.n_rate;
     if (j < a.j_sr) {
@@ -329,44 +324,49 @@ FH_EXPORT int launch_drive_out(void** p, int n, int n_spk, int batch, float gain
     FH_LAUNCH_CHECK();
 }
 
-// The frame's optic inputs, fused (flyverse/optic.py:496-517, 572, 579-580):
-// p = [rad (B, n_col, 4), sens, pr_col, cell_ptr, cell_pr, cell_w, I_lp, I_mean, fresh uint8 (B), coefs (6),
-//      contrast, a_pr (B, n_pr), rate (B, n), spk_idx int32 (n_spk), s (B, n_spk), v (B, n_rate), b (n_rate),
-//      dr (B, n_rate), W_rp group_row, chunk_ptr, pidx, pw, W_rs group_row, chunk_ptr, pidx, pw,
-//      pr_in (B, n_rate), spk_in (B, n_rate), tickets uint32 (4, zero before the first launch)];
-// dims = [n_col, n_pr, n, n_spk, n_rate, batch, W_rp groups, W_rs groups, products (0: a frame without substeps)].
-// Chunk-padded grouped storage as launch_spmv_grouped.  Launches that share `tickets` must not overlap.
+// The frame's optic inputs (flyverse/optic.py:496-517, 572, 579-580), two launches: the photoreceptor cells, feedback
+// and first dr, then (products != 0) the held inputs.  p = [rad (B, n_col, 4), sens, pr_col, cell_ptr, cell_pr,
+// cell_w, I_lp, I_mean, fresh uint8 (B), coefs (6), contrast, a_pr (B, n_pr), rate (B, n), spk_idx int32 (n_spk),
+// s (B, n_spk), v (B, n_rate), b (n_rate), dr (B, n_rate), W_rp group_row, chunk_ptr, pidx, pw, W_rs group_row,
+// chunk_ptr, pidx, pw, pr_in (B, n_rate), spk_in (B, n_rate)]; dims = [n_col, n_pr, n, n_spk, n_rate, batch,
+// W_rp groups, W_rs groups, products].  Chunk-padded grouped storage as launch_spmv_grouped.  fresh is read, not
+// cleared: launch_frame_out clears it.
 FH_EXPORT int launch_frame_in(void** p, const int* dims, float gain_in, float gain_fb, void* stream) {
     for (int i = 0; i < 9; ++i)
         if (dims[i] < 0) return FH_ERR_ARGS;
+    if (dims[5] > 65535) return FH_ERR_ARGS;
     FrameIn a;
     a.pc = PrCells{(const float*)p[0], (const float*)p[1], (const int*)p[2], (const int*)p[3], (const int*)p[4],
                    (const float*)p[5]};
-    a.I_lp = (float*)p[6]; a.I_mean = (float*)p[7]; a.fresh = (uint8_t*)p[8]; a.cf = (const float*)p[9];
+    a.I_lp = (float*)p[6]; a.I_mean = (float*)p[7]; a.fresh = (const uint8_t*)p[8]; a.cf = (const float*)p[9];
     a.contrast = (float*)p[10]; a.a_pr = (float*)p[11]; a.rate = (const float*)p[12]; a.spk_idx = (const int*)p[13];
     a.s = (float*)p[14]; a.v = (const float*)p[15]; a.bvec = (const float*)p[16]; a.dr = (float*)p[17];
     a.rp = fh_groups::Grouped{(const int*)p[18], (const int*)p[19], (const int4*)p[20], (const float4*)p[21], dims[6]};
     a.rs = fh_groups::Grouped{(const int*)p[22], (const int*)p[23], (const int4*)p[24], (const float4*)p[25], dims[7]};
-    a.pr_in = (float*)p[26]; a.spk_in = (float*)p[27]; a.tickets = (uint32_t*)p[28];
+    a.pr_in = (float*)p[26]; a.spk_in = (float*)p[27];
     a.gain_in = gain_in; a.gain_fb = gain_fb;
-    a.n_col = dims[0]; a.n_pr = dims[1]; a.n = dims[2]; a.n_spk = dims[3]; a.n_rate = dims[4]; a.batch = dims[5];
+    a.n_col = dims[0]; a.n_pr = dims[1]; a.n = dims[2]; a.n_spk = dims[3]; a.n_rate = dims[4];
+    const int batch = dims[5];
     a.j_cells = (int)nblk((int64_t)a.n_col * 5);
     a.j_spk = (int)nblk(a.n_spk);
     a.j_rate = (int)nblk(a.n_rate);
-    a.j_rp = dims[8] ? (a.rp.ngroups + 3) / 4 : 0;
-    a.j_rs = dims[8] ? (a.rs.ngroups + 3) / 4 : 0;
-    const int64_t grid = (int64_t)(a.j_cells + a.j_spk + a.j_rate + a.j_rp + a.j_rs) * a.batch;
-    if (grid >= (1ll << 31)) return FH_ERR_SIZE;
-    if (!grid) return 0;
-    hipLaunchKernelGGL(k_frame_in, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, a);
+    a.j_rp = (a.rp.ngroups + 3) / 4;
+    const int g1 = a.j_cells + a.j_spk + a.j_rate, g2 = a.j_rp + (a.rs.ngroups + 3) / 4;
+    if (!batch) return 0;
+    if (g1) {
+        hipLaunchKernelGGL(k_frame_cells, dim3(g1, batch), dim3(256), 0, (hipStream_t)stream, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
+    }
+    if (dims[8] && g2) hipLaunchKernelGGL(k_frame_held, dim3(g2, batch), dim3(256), 0, (hipStream_t)stream, a);
     FH_LAUNCH_CHECK();
 }
 
 // The frame's optic output into the LIF drive, fused (flyverse/optic.py:599-604):
 // p = [W_sr group_row, chunk_ptr, pidx, pw, v (B, n_rate), b (n_rate), dr (B, n_rate), spk_idx int32 (n_spk),
-//      drive (B, n)]; dims = [n, n_spk, n_rate, batch, W_sr groups, ready].  ready: dr holds clip01(v + b) - b
-// already (the last substep's output); otherwise the launch forms it from v and stores it.  Only the spiking targets
-// of drive are written.
+//      drive (B, n), fresh uint8 (B)]; dims = [n, n_spk, n_rate, batch, W_sr groups, ready].  ready: dr holds
+// clip01(v + b) - b already (the last substep's output); otherwise the launch forms it from v and stores it.  Only
+// the spiking targets of drive are written.  Clears fresh.
 FH_EXPORT int launch_frame_out(void** p, const int* dims, float gain_out, float clip, void* stream) {
     for (int i = 0; i < 6; ++i)
         if (dims[i] < 0) return FH_ERR_ARGS;
@@ -374,12 +374,12 @@ FH_EXPORT int launch_frame_out(void** p, const int* dims, float gain_out, float 
     FrameOut a;
     a.sr = fh_groups::Grouped{(const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3], dims[4]};
     a.v = (const float*)p[4]; a.bvec = (const float*)p[5]; a.dr = (float*)p[6]; a.spk_idx = (const int*)p[7];
-    a.drive = (float*)p[8]; a.gain_out = gain_out; a.clip = clip;
-    a.n = dims[0]; a.n_spk = dims[1]; a.n_rate = dims[2];
+    a.drive = (float*)p[8]; a.fresh = (uint8_t*)p[9]; a.gain_out = gain_out; a.clip = clip;
+    a.n = dims[0]; a.n_spk = dims[1]; a.n_rate = dims[2]; a.batch = dims[3];
     a.j_sr = (a.sr.ngroups + 3) / 4;
     const bool ready = dims[5] != 0;
-    const int gx = a.j_sr + (ready ? 0 : (int)nblk(a.n_rate));
-    if (!gx || !dims[3]) return 0;
+    const int gx = max(1, a.j_sr + (ready ? 0 : (int)nblk(a.n_rate)));
+    if (!dims[3]) return 0;
     if (ready)
         hipLaunchKernelGGL(k_frame_out<true>, dim3(gx, dims[3]), dim3(256), 0, (hipStream_t)stream, a);
     else

@@ -123,23 +123,22 @@ int launch_feedback(void** p, int n, int n_spk, int batch, void* stream);
 /* drive[:, spk_idx] = clamp(gain_out * y_sr, -clip, clip), flyverse/optic.py:602.  p = [y_sr, spk_idx, drive]. */
 int launch_drive_out(void** p, int n, int n_spk, int batch, float gain_out, float clip, void* stream);
 
-/* The frame's optic inputs in one launch (flyverse/optic.py:496-517 photoreceptor stage, :572 feedback, :579-580 the
- * held inputs): phase 1 = the photoreceptor cells (as launch_photoreceptor), s = clamp(rate[:, spk_idx] / 100, 0, 3)
- * and the first substep's dr = clip(v + b, 0, 1) - b; phase 2 (products != 0) = pr_in = gain_in * W_rp a_pr and
- * spk_in = gain_fb * W_rs s over what phase 1 wrote (grouped storage and order of launch_spmv_grouped).  Phase-2
- * workgroups take their jobs from a device ticket counter after every phase-1 job, so their wait ends; the counter
- * words are reset by the launch's last workgroup (word 3 is set if a wait ever ran past its bound).
+/* The frame's optic inputs (flyverse/optic.py:496-517 photoreceptor stage, :572 feedback, :579-580 the held
+ * inputs) in two launches: the photoreceptor cells (as launch_photoreceptor), s = clamp(rate[:, spk_idx] / 100, 0, 3)
+ * and the first substep's dr = clip(v + b, 0, 1) - b in one; then (products != 0) pr_in = gain_in * W_rp a_pr and
+ * spk_in = gain_fb * W_rs s (grouped storage and order of launch_spmv_grouped).
  * p = [rad (batch, n_col, 4), sens, pr_col, cell_ptr, cell_pr, cell_w, I_lp, I_mean, fresh uint8 (batch), coefs (6),
  * contrast, a_pr (batch, n_pr), rate (batch, n), spk_idx int32 (n_spk), s (batch, n_spk), v (batch, n_rate),
  * b (n_rate), dr (batch, n_rate), W_rp group_row, chunk_ptr, pidx, pw, W_rs group_row, chunk_ptr, pidx, pw, pr_in,
- * spk_in (batch, n_rate), tickets uint32 (4, zero before the first launch)]; dims = [n_col, n_pr, n, n_spk, n_rate,
- * batch, W_rp groups, W_rs groups, products].  Clears fresh.  Launches sharing `tickets` must not overlap. */
+ * spk_in (batch, n_rate)]; dims = [n_col, n_pr, n, n_spk, n_rate, batch, W_rp groups, W_rs groups, products].
+ * fresh is read here and cleared by launch_frame_out. */
 int launch_frame_in(void** p, const int* dims, float gain_in, float gain_fb, void* stream);
 
 /* The frame's optic output in one launch (flyverse/optic.py:599-604): drive[:, spk_idx] = clamp(gain_out * W_sr dr,
  * -clip, clip) with dr = clip(v + b, 0, 1) - b = rates() - r0.  p = [W_sr group_row, chunk_ptr, pidx, pw, v, b, dr
- * (batch, n_rate), spk_idx, drive (batch, n)]; dims = [n, n_spk, n_rate, batch, W_sr groups, ready]: ready when dr
- * already holds that value (the last substep's output), else the launch forms it from v and stores it in dr. */
+ * (batch, n_rate), spk_idx, drive (batch, n), fresh uint8 (batch)]; dims = [n, n_spk, n_rate, batch, W_sr groups,
+ * ready]: ready when dr already holds that value (the last substep's output), else the launch forms it from v and
+ * stores it in dr.  Clears fresh. */
 int launch_frame_out(void** p, const int* dims, float gain_out, float clip, void* stream);
 
 /* launch_lif_rng that also writes the transmitted 0/1 spikes as packed bits: p[17] = uint64 words (batch, ceil(n/64)),
@@ -202,11 +201,11 @@ int launch_feedback_fm(void** p, int n_spk, void* stream);
 /* drive[spk_row] = clamp(gain_out * y_sr, -clip, clip).  p = [y_sr (n_spk, 64), spk_row, drive (rows, 64)]. */
 int launch_drive_out_fm(void** p, int n_spk, float gain_out, float clip, void* stream);
 
-/* launch_frame_in in the fly-minor layout: phase 1 = launch_photoreceptor_fm, launch_feedback_fm and the first dr;
- * phase 2 (grid > 0) = pr_in = gain_in * W_rp a_pr and spk_in = gain_fb * W_rs s, each streamed by grid workgroups
+/* launch_frame_in in the fly-minor layout: launch_photoreceptor_fm, launch_feedback_fm and the first dr in one launch;
+ * then (grid > 0) pr_in = gain_in * W_rp a_pr and spk_in = gain_fb * W_rs s, each streamed by grid workgroups
  * (flyverse/optic.py:496-517, 572, 579-580).  p = [the 13 pointers of launch_photoreceptor_fm, rate (rows, 64),
  * spk_row, s (n_spk, 64), v, b, dr (n_rate, 64), W_rp ptr, idx, w, sched, W_rs ptr, idx, w, sched, pr_in, spk_in
- * (n_rate, 64), tickets uint32 (4)]; dims = [n_cells, n_spk, n_rate, grid]. */
+ * (n_rate, 64)]; dims = [n_cells, n_spk, n_rate, grid]. */
 int launch_frame_in_fm(void** p, const int* dims, float gain_in, float gain_fb, void* stream);
 
 /* drive[spk_row] = clamp(gain_out * W_sr dr, -clip, clip) in one launch, dr (n_rate, 64) the last substep's output
