@@ -57,6 +57,45 @@ __global__ __launch_bounds__(256) void k_optic_update(OpticArgs s, int n, int64_
     s.dr[t] = fh_clip01(q) - s.b[i];
 }
 
+// One substep fused with its recurrent product (flyverse/optic.py:474-483 then 525-551): y = W_rr dr_cur for the
+// rows of group g (grouped order of launch_spmv_grouped), then each row's update as k_optic_update computes it, the
+// next dr into dr_next (a different buffer: other rows' products still read dr_cur).  One wavefront per (group, fly).
+struct SubstepArgs {
+    fh_groups::Grouped rr;
+    const float* dr_cur;
+    float *dr_next, *v, *adapt;
+    const float *pr, *spk, *a, *b;
+    int n_rate;
+    float gr, ga, aad;
+};
+
+__global__ __launch_bounds__(256) void k_optic_substep(SubstepArgs s) {
+    __shared__ float sums[4][kWave];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = blockIdx.x * 4 + wv;
+    if (g >= s.rr.ngroups) return;
+    const int64_t base = (int64_t)blockIdx.y * s.n_rate;
+    const float* x = s.dr_cur + base;
+    fh_groups::group_product(s.rr, g, lane, sums[wv], [&](int i) { return x[i]; }, [&](int i, float total) {
+        const int64_t t = base + i;
+        const float old = s.dr_cur[t];
+        float inp = s.gr * total;
+        inp = inp + s.pr[t];
+        float gad = s.ga * s.adapt[t];
+        inp = inp - gad;
+        inp = inp + s.spk[t];
+        float d = s.v[t] - inp;
+        d = d * s.a[i];
+        float v = inp + d;
+        s.v[t] = v;
+        float da = s.adapt[t] - old;
+        da = da * s.aad;
+        s.adapt[t] = old + da;
+        s.dr_next[t] = dr_value(v, s.b[i]);
+    });
+}
+
 // Photoreceptor stage over (column, family) cells (flyverse/optic.py:496-517), one thread per (fly b, cell k),
 // cell = column * 5 + family.  The cell's input is its row of the averaging matrix (optic.py:320-325: weights
 // cell_w = 1/count, photoreceptors ascending) applied to the intensities I_p = sum_c rad[b, column(p), c] * sens[p, c]
@@ -210,13 +249,14 @@ __global__ __launch_bounds__(256) void k_frame_held(FrameIn a) {
 }
 
 // Frame end (flyverse/optic.py:599-604): drive[b, spk_idx[k]] = clamp(gain_out * (W_sr dr)[k]) with
-// dr = clip01(v + b) - b, the rate change from r0 = b.  READY: dr is the last substep's output, read as is (the substep
-// writes exactly this value); otherwise (a frame without substeps) the product forms dr from v as it gathers, and
-// the launch's own rate-unit jobs store it.  No job reads what another writes.  Workgroup (0, 0) also clears the
-// fresh flags, which the frame start read.
+// dr = clip01(v + b) - b, the rate change from r0 = b.  READY: dr_src is the last substep's output, read as is (the
+// substep writes exactly this value), and copied into dr when it is another buffer; otherwise (a frame without
+// substeps) the product forms dr from v as it gathers, and the launch's own rate-unit jobs store it.  No job reads
+// what another writes.  Workgroup (0, 0) also clears the fresh flags, which the frame start read.
 struct FrameOut {
     fh_groups::Grouped sr;
     const float *v, *bvec;  // (B, n_rate), (n_rate)
+    const float* dr_src;    // (B, n_rate): the last substep's output (READY)
     float* dr;              // (B, n_rate)
     const int* spk_idx;     // (n_spk)
     float* drive;           // (B, n)
@@ -232,6 +272,7 @@ __global__ __launch_bounds__(256) void k_frame_out(FrameOut a) {
     if (b == 0 && j == 0)
         for (int i = threadIdx.x; i < a.batch; i += blockDim.x) a.fresh[i] = 0;
     const float* vb = a.v + (int64_t)b * a.n_rate;
+    const float* srcb = a.dr_src + (int64_t)b * a.n_rate;
     float* drb = a.dr + (int64_t)b * a.n_rate;
     if (j < a.j_sr) {
         const int lane = threadIdx.x & 63;
@@ -241,13 +282,13 @@ __global__ __launch_bounds__(256) void k_frame_out(FrameOut a) {
         float* drive = a.drive + (int64_t)b * a.n;
         auto finish = [&](int row, float tot) { drive[a.spk_idx[row]] = drive_value(tot, a.gain_out, a.clip); };
         if (READY)
-            fh_groups::group_product(a.sr, g, lane, sums[wv], [&](int i) { return drb[i]; }, finish);
+            fh_groups::group_product(a.sr, g, lane, sums[wv], [&](int i) { return srcb[i]; }, finish);
         else
             fh_groups::group_product(a.sr, g, lane, sums[wv], [&](int i) { return dr_value(vb[i], a.bvec[i]); },
                                      finish);
-    } else if (!READY) {
+    } else {
         const int r = (j - a.j_sr) * 256 + threadIdx.x;
-        if (r < a.n_rate) drb[r] = dr_value(vb[r], a.bvec[r]);
+        if (r < a.n_rate) drb[r] = READY ? srcb[r] : dr_value(vb[r], a.bvec[r]);
     }
 }
 
@@ -285,6 +326,21 @@ FH_EXPORT int launch_optic(void** p, int n, int batch, float gr, float ga, float
     OpticArgs s{(float*)p[0], (float*)p[1], (float*)p[2], (const float*)p[3], (const float*)p[4],
                 (const float*)p[5], (const float*)p[6], (const float*)p[7]};
     hipLaunchKernelGGL(k_optic_update, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, s, n, total, gr, ga, aad);
+    FH_LAUNCH_CHECK();
+}
+
+// One substep fused with its recurrent product: p = [W_rr group_row, chunk_ptr, pidx, pw (grouped storage of
+// launch_spmv_grouped), dr_cur, dr_next, v, adapt, pr, spk (B, n_rate each), a, b (n_rate)]; ngroups = W_rr groups.
+// The product of launch_spmv_grouped (scale 1) followed by launch_optic's update, writing the next dr into dr_next.
+FH_EXPORT int launch_optic_substep(void** p, int n_rate, int ngroups, int batch, float gr, float ga, float aad,
+                                   void* stream) {
+    if (n_rate < 0 || ngroups < 0 || batch < 0 || batch > 65535) return FH_ERR_ARGS;
+    if (!n_rate || !ngroups || !batch) return 0;
+    SubstepArgs s{fh_groups::Grouped{(const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3],
+                                     ngroups},
+                  (const float*)p[4], (float*)p[5], (float*)p[6], (float*)p[7], (const float*)p[8],
+                  (const float*)p[9], (const float*)p[10], (const float*)p[11], n_rate, gr, ga, aad};
+    hipLaunchKernelGGL(k_optic_substep, dim3((ngroups + 3) / 4, batch), dim3(256), 0, (hipStream_t)stream, s);
     FH_LAUNCH_CHECK();
 }
 
@@ -364,9 +420,10 @@ FH_EXPORT int launch_frame_in(void** p, const int* dims, float gain_in, float ga
 
 // The frame's optic output into the LIF drive, fused (flyverse/optic.py:599-604):
 // p = [W_sr group_row, chunk_ptr, pidx, pw, v (B, n_rate), b (n_rate), dr (B, n_rate), spk_idx int32 (n_spk),
-//      drive (B, n), fresh uint8 (B)]; dims = [n, n_spk, n_rate, batch, W_sr groups, ready].  ready: dr holds
-// clip01(v + b) - b already (the last substep's output); otherwise the launch forms it from v and stores it.  Only
-// the spiking targets of drive are written.  Clears fresh.
+//      drive (B, n), fresh uint8 (B), dr_src (B, n_rate)]; dims = [n, n_spk, n_rate, batch, W_sr groups, ready].
+// ready: dr_src holds clip01(v + b) - b already (the last substep's output) and is copied into dr if it is another
+// buffer; otherwise the launch forms it from v and stores it in dr (dr_src unused).  Only the spiking targets of
+// drive are written.  Clears fresh.
 FH_EXPORT int launch_frame_out(void** p, const int* dims, float gain_out, float clip, void* stream) {
     for (int i = 0; i < 6; ++i)
         if (dims[i] < 0) return FH_ERR_ARGS;
@@ -374,11 +431,13 @@ FH_EXPORT int launch_frame_out(void** p, const int* dims, float gain_out, float 
     FrameOut a;
     a.sr = fh_groups::Grouped{(const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3], dims[4]};
     a.v = (const float*)p[4]; a.bvec = (const float*)p[5]; a.dr = (float*)p[6]; a.spk_idx = (const int*)p[7];
-    a.drive = (float*)p[8]; a.fresh = (uint8_t*)p[9]; a.gain_out = gain_out; a.clip = clip;
+    a.drive = (float*)p[8]; a.fresh = (uint8_t*)p[9]; a.dr_src = (const float*)p[10];
+    a.gain_out = gain_out; a.clip = clip;
     a.n = dims[0]; a.n_spk = dims[1]; a.n_rate = dims[2]; a.batch = dims[3];
     a.j_sr = (a.sr.ngroups + 3) / 4;
     const bool ready = dims[5] != 0;
-    const int gx = max(1, a.j_sr + (ready ? 0 : (int)nblk(a.n_rate)));
+    const bool store = !ready || a.dr_src != a.dr;
+    const int gx = max(1, a.j_sr + (store ? (int)nblk(a.n_rate) : 0));
     if (!dims[3]) return 0;
     if (ready)
         hipLaunchKernelGGL(k_frame_out<true>, dim3(gx, dims[3]), dim3(256), 0, (hipStream_t)stream, a);

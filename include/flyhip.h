@@ -123,6 +123,12 @@ int launch_feedback(void** p, int n, int n_spk, int batch, void* stream);
 /* drive[:, spk_idx] = clamp(gain_out * y_sr, -clip, clip), flyverse/optic.py:602.  p = [y_sr, spk_idx, drive]. */
 int launch_drive_out(void** p, int n, int n_spk, int batch, float gain_out, float clip, void* stream);
 
+/* One optic substep with its recurrent product fused (flyverse/optic.py:474-483 then 525-551; neural.cu:68-79 for
+ * the update): y = W_rr dr_cur in the grouped order of launch_spmv_grouped, then launch_optic's update, the next dr
+ * into dr_next (another buffer).  p = [W_rr group_row, chunk_ptr, pidx, pw, dr_cur, dr_next, v, adapt, pr, spk
+ * (batch, n_rate each), a, b (n_rate)]; ngroups = W_rr groups.  Replaces a launch_csr + launch_optic pair. */
+int launch_optic_substep(void** p, int n_rate, int ngroups, int batch, float gr, float ga, float aad, void* stream);
+
 /* The frame's optic inputs (flyverse/optic.py:496-517 photoreceptor stage, :572 feedback, :579-580 the held
  * inputs) in two launches: the photoreceptor cells (as launch_photoreceptor), s = clamp(rate[:, spk_idx] / 100, 0, 3)
  * and the first substep's dr = clip(v + b, 0, 1) - b in one; then (products != 0) pr_in = gain_in * W_rp a_pr and
@@ -136,9 +142,9 @@ int launch_frame_in(void** p, const int* dims, float gain_in, float gain_fb, voi
 
 /* The frame's optic output in one launch (flyverse/optic.py:599-604): drive[:, spk_idx] = clamp(gain_out * W_sr dr,
  * -clip, clip) with dr = clip(v + b, 0, 1) - b = rates() - r0.  p = [W_sr group_row, chunk_ptr, pidx, pw, v, b, dr
- * (batch, n_rate), spk_idx, drive (batch, n), fresh uint8 (batch)]; dims = [n, n_spk, n_rate, batch, W_sr groups,
- * ready]: ready when dr already holds that value (the last substep's output), else the launch forms it from v and
- * stores it in dr.  Clears fresh. */
+ * (batch, n_rate), spk_idx, drive (batch, n), fresh uint8 (batch), dr_src (batch, n_rate)]; dims = [n, n_spk, n_rate,
+ * batch, W_sr groups, ready]: ready when dr_src already holds that value (the last substep's output; copied into dr
+ * when it is another buffer), else the launch forms it from v and stores it in dr.  Clears fresh. */
 int launch_frame_out(void** p, const int* dims, float gain_out, float clip, void* stream);
 
 /* launch_lif_rng that also writes the transmitted 0/1 spikes as packed bits: p[17] = uint64 words (batch, ceil(n/64)),
