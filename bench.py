@@ -11,7 +11,8 @@ Shiu LIF at dt 0.1 ms coupled to the 89,390-unit optic lobe every 10 ms frame (1
     frame shape; reported as b1_sim_s_per_wall_s and the ratio.
   * roofline: the batched LIF launch (csrc/batch.hip k_batch_lif, <= n_delay steps per launch): one frame of its
     launches timed with HIP events on the launch stream right after the timed region; algorithmic bytes per launch
-    from the per-step LIF and synaptic-input byte counts (lif_block_bytes).
+    from the per-step LIF and synaptic-input byte counts (lif_block_bytes).  The fused optic substep of the same
+    engine (k_optic_substep_fm) is timed the same way and reported as optic_substep_b64_us / _frac.
   * packed: 128 LIF subgraphs (5k-80k cells, log-uniform; in-degree of the full graph) packed into one CSR and
     stepped 5,000 steps by one replay of a captured graph of window launches (csrc/packed.hip), window and grid
     from flyhip.autotune.
@@ -106,6 +107,35 @@ def time_lif_launches(bb):
         times.append(e0.elapsed_time(e1))
         ks.append(K)
     return float(np.mean(times)), float(np.mean(ks))
+
+
+def time_optic_substeps(bb, reps=20):
+    """The fused fly-minor optic substep (csrc/batch.hip k_optic_substep_fm: W_rr product + update for 64 flies),
+    `reps` launches timed with HIP events on the launch stream, over a copy of the engine's optic state (the engine's
+    own state is left as it was).  Returns (mean us, algorithmic bytes 8 nnz + 4 (rows + 1) + 64 rows 4 B x 8)."""
+    import ctypes as ct
+    import torch
+    from flyhip.batch import _lib
+    M = bb.W_rr
+    dr0, dr1, v, ad = (t.clone() for t in (bb.dr0, bb.dr1, bb.ov, bb.oadapt))
+    ts = [M.ptr, M.idx, M.w, M.sched, dr0, dr1, v, ad, bb.pr_in, bb.spk_in, bb.o_a, bb.o_b, bb.live_mask]
+    arr = (ct.c_void_p * len(ts))(*(t.data_ptr() for t in ts))
+    fn = _lib(bb.device).launch_optic_substep_fm
+    stream = torch.cuda.current_stream()
+    args = (bb.n_rate, float(bb.op.gain_rr), float(bb.op.adapt_gain), float(bb.o_a_ad), bb.optic_grid(bb.n_rate),
+            stream.cuda_stream)
+    fn(arr, *args)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        err = fn(arr, *args)
+        if err:
+            raise RuntimeError(f"launch_optic_substep_fm: {err}")
+    e1.record(stream)
+    e1.synchronize()
+    us = e0.elapsed_time(e1) * 1000 / reps
+    nnz = int(M.nnz)
+    return us, 8 * nnz + 4 * (bb.n_rate + 1) + 64 * bb.n_rate * 4 * 8
 
 
 def spmv_roofline(conn, retina, op, reps=50):
@@ -419,6 +449,9 @@ def main():
     achieved = alg_bytes / (lif_ms / 1e3) / 1e9
     log(f"k_batch_lif: {lif_ms:.3f} ms per launch ({K:.3f} steps on average), activity {activity*100:.3f} % per "
         f"step, {alg_bytes/1e6:.1f} MB algorithmic -> {achieved:.0f} GB/s")
+    sub_us, sub_bytes = time_optic_substeps(bb)
+    sub_frac = sub_bytes / (sub_us * 1e-6) / 1e9 / HBM_PEAK_GBS
+    log(f"k_optic_substep_fm: {sub_us:.1f} us, {sub_bytes / 1e6:.1f} MB algorithmic -> {sub_frac * 100:.1f} % of peak")
     if tuned is None:
         tuned = {"fold_mode": bb.fold_mode, "steps_per_launch": bb.steps_per_launch,
                  "grid_per_cu": bb.grid // torch.cuda.get_device_properties(dev).multi_processor_count}
@@ -428,7 +461,9 @@ def main():
     if args.only == "b64":
         if args.copy_calibration:
             copy_calibration(dev)
-        print(json.dumps({"b64_sim_s_per_wall_s": round(value, 4), "lif_launch_ms": round(lif_ms, 4)}), flush=True)
+        print(json.dumps({"b64_sim_s_per_wall_s": round(value, 4), "lif_launch_ms": round(lif_ms, 4),
+                          "optic_substep_b64_us": round(sub_us, 2), "optic_substep_b64_frac": round(sub_frac, 4)}),
+              flush=True)
         return
 
     spmv = spmv_roofline(conn, retina, op)
@@ -484,7 +519,8 @@ def main():
                    "batch": 64, "neurons": int(conn.n), "edges": int(conn.W.nnz), "dt_ms": lp.dt,
                    "b1_sim_s_per_wall_s": b1, "ratio_b64_vs_b1": round(value / b1, 2),
                    "b1_ms_per_frame": b1_ms_frame, "spike_activity_per_step": activity,
-                   "b64_launch_shape": tuned, **spmv, **(ol or {}), **(packed or {})},
+                   "b64_launch_shape": tuned, "optic_substep_b64_us": round(sub_us, 2),
+                   "optic_substep_b64_frac": round(sub_frac, 4), **spmv, **(ol or {}), **(packed or {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_batch_lif", "launch_ms": round(lif_ms, 4), "steps_per_launch": round(K, 4),
