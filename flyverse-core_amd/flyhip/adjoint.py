@@ -144,7 +144,12 @@ class OpticAdjoint:
         self.ck_v = z(nseg, n, NL)
         self.ck_a = z(nseg, n, NL)
         C = self.checkpoint
-        self.S_v, self.S_a, self.S_dr, self.S_inp = z(C + 1, n, NL), z(C + 1, n, NL), z(C + 1, n, NL), z(C, n, NL)
+        seg = lambda: (z(C + 1, n, NL), z(C + 1, n, NL), z(C + 1, n, NL), z(C, n, NL))
+        # a segment's states and inputs: set 0 holds the forward's last segment; with several segments a second set
+        # lets the backward of one segment overlap the recompute of the one before it (on a side stream)
+        self._sets = [seg()] + ([seg()] if nseg > 1 else [])
+        self.S_v, self.S_a, self.S_dr, self.S_inp = self._sets[0]
+        self._side = torch.cuda.Stream(device=dev) if nseg > 1 else None
         self.lv, self.la, self.ly = z(n, NL), z(n, NL), z(n, NL)
         self.ga, self.gb, self.eg = z(n), z(n), z(len(bidx))
         self._params = None
@@ -227,7 +232,12 @@ class OpticAdjoint:
 
     # ------------------------------------------------------------------ backward
     def gradients(self):
-        """(d loss/d tau, d loss/d bias, d loss/d gain) at the parameters of the last `loss` call."""
+        """(d loss/d tau, d loss/d bias, d loss/d gain) at the parameters of the last `loss` call.
+
+        Segments are walked last to first.  Each one but the forward's last is recomputed from its checkpoint (the
+        forward's launches: the same bits) into one of two segment buffers, on a side stream, while the backward of
+        the segment after it runs on the current stream from the other buffer; events order a buffer's reuse after
+        the backward that read it.  The backward's launches, and so its additions, are in the same order either way."""
         if self._params is None:
             raise RuntimeError("call loss() first")
         n, C, T = self.n, self.checkpoint, self.T
@@ -236,28 +246,52 @@ class OpticAdjoint:
         for buf in (self.lv, self.la, self.ga, self.gb, self.eg):
             buf.zero_()
         nseg = (T + C - 1) // C
-        first = True
-        for s in reversed(range(nseg)):
+        main = torch.cuda.current_stream(self.device)
+        if self._side is not None:
+            self._side.wait_stream(main)         # the forward's checkpoints and parameters first
+        freed = [None, None]                     # per buffer set: the event after the last backward that read it
+        ready = {}                               # per segment: the event after its recompute
+
+        def recompute(s, which):
             t0, t1 = s * C, min(T, (s + 1) * C)
-            if not (first and self._last_segment_ready):
-                # recompute the segment's states from its checkpoint (the same launches as the forward: same bits)
-                self.S_v[0].copy_(self.ck_v[s])
-                self.S_a[0].copy_(self.ck_a[s])
-                self._call("launch_adj_dr", [self.S_v[0], self.b, self.S_dr[0]], n)
+            S_v, S_a, S_dr, S_inp = self._sets[which]
+            stream = self._side if self._side is not None else main
+            with torch.cuda.stream(stream):
+                if freed[which] is not None:
+                    stream.wait_event(freed[which])
+                S_v[0].copy_(self.ck_v[s])
+                S_a[0].copy_(self.ck_a[s])
+                self._call("launch_adj_dr", [S_v[0], self.b, S_dr[0]], n)
                 for k in range(t1 - t0):
-                    self._step(t0 + k, self.S_v[k], self.S_a[k], self.S_dr[k], self.S_v[k + 1], self.S_a[k + 1],
-                               self.S_dr[k + 1], self.S_inp[k])
-            self._last_segment_ready = False
-            if first:
+                    self._step(t0 + k, S_v[k], S_a[k], S_dr[k], S_v[k + 1], S_a[k + 1], S_dr[k + 1], S_inp[k])
+                ev = torch.cuda.Event()
+                ev.record(stream)
+            ready[s] = ev
+
+        last = nseg - 1
+        if not self._last_segment_ready:
+            recompute(last, 0)
+        self._last_segment_ready = False
+        for s in reversed(range(nseg)):
+            which = (last - s) % 2
+            if s > 0:
+                recompute(s - 1, (last - s + 1) % 2)        # runs beside this segment's backward
+            if s in ready:
+                main.wait_event(ready[s])
+            S_v, S_a, S_dr, S_inp = self._sets[which]
+            t0, t1 = s * C, min(T, (s + 1) * C)
+            if s == last:
                 # the loss term of the last step, T: lambda_v_T = loss_scale (r_T - target_T) gate_T
                 self.ly.zero_()
-                self._back_rows(t1 - t0, T, f_back)
-                first = False
+                self._back_rows(t1 - t0, T, f_back, None, S_v, S_dr)
             for k in reversed(range(t1 - t0)):
                 t = t0 + k
-                self._call("launch_adj_back_ly", [self.lv, self.S_v[k], self.S_inp[k], self.a, self.ly, self.ga], n,
+                self._call("launch_adj_back_ly", [self.lv, S_v[k], S_inp[k], self.a, self.ly, self.ga], n,
                            float(self.c.gain_rr))
-                self._back_rows(k, t, f_back, self.eg)
+                self._back_rows(k, t, f_back, self.eg, S_v, S_dr)
+            ev = torch.cuda.Event()
+            ev.record(main)
+            freed[which] = ev
         tau, bias, gain = self._params
         K = self.K
         ga_t, gb_t, gg = (torch.empty(K, device=self.device), torch.empty(K, device=self.device),
@@ -269,12 +303,15 @@ class OpticAdjoint:
         g_tau = ga_t * a_k * (self.c.dt / (tau * tau))
         return g_tau, gb_t, gg.view(K, K)
 
-    def _back_rows(self, k, t, f_back, eg=None):
-        """The transposed product and the rest of step t's adjoint; with eg, the same gathers also add each edge's
-        sum over movies of ly[row] * dr_t[col] to eg (chunk-padded W^T order)."""
+    def _back_rows(self, k, t, f_back, eg=None, S_v=None, S_dr=None):
+        """The transposed product and the rest of step t's adjoint (states of step t: row k of the segment buffers
+        S_v, S_dr, by default the first set); with eg, the same gathers also add each edge's sum over movies of
+        ly[row] * dr_t[col] to eg (chunk-padded W^T order)."""
+        S_v = self.S_v if S_v is None else S_v
+        S_dr = self.S_dr if S_dr is None else S_dr
         target = self.target[t - 1] if t >= 1 else None
         self._call("launch_adj_back_rows",
-                   [self.bptr, self.bidx, self.schedT, self.bw, self.a, self.b, self.ly, self.S_v[k], self.S_dr[k],
+                   [self.bptr, self.bidx, self.schedT, self.bw, self.a, self.b, self.ly, S_v[k], S_dr[k],
                     self.lv, self.la, self.readout, target, self.gb, eg],
                    self.n, f_back, self.grid)
 
