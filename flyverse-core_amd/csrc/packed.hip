@@ -123,7 +123,11 @@ __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
             const int e0 = __builtin_amdgcn_readlane(pl0, i), e1 = __builtin_amdgcn_readlane(pl1, i);
             const float g0 = fh_readlane(gl, i);
             float gk;
-            if (e1 - e0 <= 4 * kWave) {
+            if (e1 - e0 <= kWave) {
+                gk = fh_chain::g_chain_cached<1>(E, W, e0, e1, g0, lane);
+            } else if (e1 - e0 <= 2 * kWave) {
+                gk = fh_chain::g_chain_cached<2>(E, W, e0, e1, g0, lane);
+            } else if (e1 - e0 <= 4 * kWave) {
                 gk = fh_chain::g_chain_cached<4>(E, W, e0, e1, g0, lane);
             } else if (e1 - e0 <= kShortEdges) {
                 gk = fh_chain::g_chain_cached<8>(E, W, e0, e1, g0, lane);

@@ -109,7 +109,7 @@ __device__ __forceinline__ void lif_window_cell(const Plain& a, uint32_t T0, int
 //  blocks      one wavefront per (fly, block of kBlock consecutive cells), blocks in `sched` order (most in-edges
 //              first).  Phase A: for each short cell of the block with in-edges, lanes over its in-edges (edges and
 //              window bits in registers, the register count fitted to the cell's chunks), the chain with each step's
-//              fired weights folded through a per-wavefront LDS list; g_k into LDS.  Phase B: lanes over the block's
+//              fired weights added in edge order from the step's ballot; g_k into LDS.  Phase B: lanes over the block's
 //              cells, the LIF updates (a cell without in-edges only decays: g_k = g_{k-1} * a_s).  Long cells are left
 //              to their own workgroups, which come first in the grid.
 // A cell's history word and state are written by the one wavefront that owns it; the bits other wavefronts read
@@ -118,19 +118,20 @@ constexpr int kBlock = 16;
 constexpr int kVlongCompact = 24;              // chunks per wavefront of a very long cell (up to 6144 in-edges)
 constexpr int kLongCompact = 8;               // chunks per wavefront of a long cell (up to 2048 in-edges)
 
-// The chain of one short cell with its edges in registers, NC fitted to its chunk count.
+// The chain of one short cell with its edges in registers, NC fitted to its chunk count.  At a few fired in-edges
+// per step, walking each step's ballot (chain_steps_bits) is cheaper than compacting it through LDS for the lane
+// fold (chain_steps_fold): B=1 frame 1.67 -> 1.47 ms.
 template <int NC>
-__device__ __forceinline__ float short_chain(const Edges& E, const Window& W, int e0, int e1, float g0, int lane,
-                                             float* list) {
+__device__ __forceinline__ float short_chain(const Edges& E, const Window& W, int e0, int e1, float g0, int lane) {
     float wc[NC];
     uint32_t hist[NC];
     fh_chain::load_window_edges<NC>(E, W, e0, e1, lane, wc, hist);
-    return fh_chain::chain_steps_fold<NC>(W, e0, e1, g0, lane, wc, hist, list);
+    return fh_chain::chain_steps_bits<NC>(W, e0, e1, g0, lane, wc, hist);
 }
 
-// The LDS of a workgroup: a long cell's four compaction lists, or a block wavefront's g_k table and fold list.
+// The LDS of a workgroup: a long cell's four compaction lists, or a block wavefront's g_k table.
 constexpr int kLongLds = 4 * (kVlongCompact * kWave + kWave);
-constexpr int kBlockLds = kBlock * kMaxWindow + kShortEdges + kWave;
+constexpr int kBlockLds = kBlock * kMaxWindow;
 static_assert(4 * kBlockLds <= kLongLds, "the block wavefronts' LDS fits in the long cells' buffer");
 
 // One launch per window: workgroups [0, long_wgs) take one (fly, long cell) each, the rest four blocks each.
@@ -166,7 +167,6 @@ __global__ __launch_bounds__(256) void k_win(Plain a, const uint32_t* step_base,
     const int nc = min(kBlock, a.n - c0);
     const Edges E = edges_of(a, fly);
     float* const gb = lds + wv * kBlockLds;
-    float* const list = gb + kBlock * kMaxWindow;
     const int pl0 = a.ptr[c0 + min(lane, nc)], pl1 = a.ptr[c0 + min(lane + 1, nc)];
     const bool is_long = lane < nc && pl1 - pl0 > kShortEdges;
     const float gl = lane < nc ? a.g[(int64_t)fly * a.n + c0 + lane] : 0.0f;
@@ -180,13 +180,13 @@ __global__ __launch_bounds__(256) void k_win(Plain a, const uint32_t* step_base,
         const int nch = (e1 - e0 + kWave - 1) / kWave;
         float gk;
         if (nch <= 1)
-            gk = short_chain<1>(E, W, e0, e1, g0, lane, list);
+            gk = short_chain<1>(E, W, e0, e1, g0, lane);
         else if (nch <= 2)
-            gk = short_chain<2>(E, W, e0, e1, g0, lane, list);
+            gk = short_chain<2>(E, W, e0, e1, g0, lane);
         else if (nch <= 4)
-            gk = short_chain<4>(E, W, e0, e1, g0, lane, list);
+            gk = short_chain<4>(E, W, e0, e1, g0, lane);
         else
-            gk = short_chain<8>(E, W, e0, e1, g0, lane, list);
+            gk = short_chain<8>(E, W, e0, e1, g0, lane);
         if (lane < K) gb[i * kMaxWindow + lane] = gk;
     }
     __builtin_amdgcn_wave_barrier();
