@@ -28,7 +28,7 @@ struct Plain {
     const int* ptr;          // (n + 1) post-major CSR shared by the flies, columns ascending
     const int* idx;          // (nnz) presynaptic cell
     const float* w;          // (nnz)
-    const int* sched;        // (ceil(n / 16)) blocks of 16 cells, most short-cell in-edges first
+    const int* sched;        // (ceil(n / 8)) blocks of 8 cells, most short-cell in-edges first
     float *v, *g, *refrac, *rate, *counts, *adapt, *spikes;   // (B, n)
     const float* drive;      // (B, n)
     const float* pois;       // (B, n) per-step Poisson probability
@@ -114,7 +114,7 @@ __device__ __forceinline__ void lif_window_cell(const Plain& a, uint32_t T0, int
 //              to their own workgroups, which come first in the grid.
 // A cell's history word and state are written by the one wavefront that owns it; the bits other wavefronts read
 // from it (steps T0 - delay .. T0 - delay + K - 1) are not among those it rewrites (T0 .. T0 + K - 1).
-constexpr int kBlock = 16;
+constexpr int kBlock = 8;
 constexpr int kVlongCompact = 24;              // chunks per wavefront of a very long cell (up to 6144 in-edges)
 constexpr int kLongCompact = 8;               // chunks per wavefront of a long cell (up to 2048 in-edges)
 
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_win(Plain a, const uint32_t* step_base,
 }  // namespace
 
 // K LIF steps of the plain (batch, n) brain from global step *step_base + step (step_base: device uint32 or null).
-// p = [ptr (n+1), idx, w (float), sched (ceil(n/16) blocks of 16 cells), v, g, refrac, rate, counts, adapt, spikes,
+// p = [ptr (n+1), idx, w (float), sched (ceil(n/8) blocks of 8 cells), v, g, refrac, rate, counts, adapt, spikes,
 //      drive, pois, active (n), spike_buf (delay, batch, n), hist uint64 (batch, n), P (11), long_rows (n_long: every
 //      cell with more than 512 in-edges, most first, the n_vlong with more than 2048 first)];
 // dims = [n, batch, K, delay, slot0 (spike_buf slot of the first step), n_long, n_vlong, flags (1 Poisson, 4 counts,
