@@ -18,7 +18,6 @@ namespace {
 
 using fh_chain::Edges;
 using fh_chain::kMaxWindow;
-using fh_chain::kShortEdges;
 using fh_chain::kSlots;
 using fh_chain::Window;
 
@@ -36,7 +35,7 @@ struct Plain {
     float* ring;             // (n_delay, B, n) spike_buf: the slot of step t holds the spikes emitted at step t
     uint64_t* hist;          // (B, n) spike history words
     const float* P;          // (11) v_rest, v_reset, v_th, a_m, dt, t_ref, adapt_jump, a_ad, -, a_r, rate_gain
-    const int* long_rows;    // (n_long) cells with more than 512 in-edges, most first
+    const int* long_rows;    // (n_long) cells with more than 1024 in-edges, most first
     uint64_t seed;
     int n, batch, delay, slot0, n_long, n_vlong, flags;
     float a_s;
@@ -103,7 +102,7 @@ __device__ __forceinline__ void lif_window_cell(const Plain& a, uint32_t T0, int
 }
 
 // A window is one launch of independent work items (no item waits on another):
-//  long cells  one workgroup per (fly, cell with more than 512 in-edges): the cell's chain over the four wavefronts
+//  long cells  one workgroup per (fly, cell with more than 1024 in-edges): the cell's chain over the four wavefronts
 //              (fh_chain::g_chain_compact: each step's fired weights compacted in edge order by the wavefronts in
 //              parallel, folded by wavefront 0 with the lane fold), then its LIF update on wavefront 0;
 //  blocks      one wavefront per (fly, block of kBlock consecutive cells), blocks in `sched` order (most in-edges
@@ -115,6 +114,10 @@ __device__ __forceinline__ void lif_window_cell(const Plain& a, uint32_t T0, int
 // A cell's history word and state are written by the one wavefront that owns it; the bits other wavefronts read
 // from it (steps T0 - delay .. T0 - delay + K - 1) are not among those it rewrites (T0 .. T0 + K - 1).
 constexpr int kBlock = 8;
+// Cells with more in-edges than this take a workgroup of their own (the host's long_rows).  One wavefront walking up
+// to 16 chunks in registers beats the four-wavefront compaction up to here: B=1 frame 1.44 ms at 512, 1.17 at 1024,
+// 1.27 at 1536 (24 chunks' registers lower the whole kernel's occupancy).
+constexpr int kWinShort = 16 * kWave;
 constexpr int kVlongCompact = 24;              // chunks per wavefront of a very long cell (up to 6144 in-edges)
 constexpr int kLongCompact = 8;               // chunks per wavefront of a long cell (up to 2048 in-edges)
 
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(256) void k_win(Plain a, const uint32_t* step_base,
     const Edges E = edges_of(a, fly);
     float* const gb = lds + wv * kBlockLds;
     const int pl0 = a.ptr[c0 + min(lane, nc)], pl1 = a.ptr[c0 + min(lane + 1, nc)];
-    const bool is_long = lane < nc && pl1 - pl0 > kShortEdges;
+    const bool is_long = lane < nc && pl1 - pl0 > kWinShort;
     const float gl = lane < nc ? a.g[(int64_t)fly * a.n + c0 + lane] : 0.0f;
     // ---------------------------------------------------------------- phase A: chains of the short cells
     uint64_t todo = __builtin_amdgcn_ballot_w64(lane < nc && pl1 > pl0 && !is_long);
@@ -185,8 +188,10 @@ __global__ __launch_bounds__(256) void k_win(Plain a, const uint32_t* step_base,
             gk = short_chain<2>(E, W, e0, e1, g0, lane);
         else if (nch <= 4)
             gk = short_chain<4>(E, W, e0, e1, g0, lane);
-        else
+        else if (nch <= 8)
             gk = short_chain<8>(E, W, e0, e1, g0, lane);
+        else
+            gk = short_chain<16>(E, W, e0, e1, g0, lane);
         if (lane < K) gb[i * kMaxWindow + lane] = gk;
     }
     __builtin_amdgcn_wave_barrier();
@@ -204,7 +209,7 @@ __global__ __launch_bounds__(256) void k_win(Plain a, const uint32_t* step_base,
 // K LIF steps of the plain (batch, n) brain from global step *step_base + step (step_base: device uint32 or null).
 // p = [ptr (n+1), idx, w (float), sched (ceil(n/8) blocks of 8 cells), v, g, refrac, rate, counts, adapt, spikes,
 //      drive, pois, active (n), spike_buf (delay, batch, n), hist uint64 (batch, n), P (11), long_rows (n_long: every
-//      cell with more than 512 in-edges, most first, the n_vlong with more than 2048 first)];
+//      cell with more than 1024 in-edges, most first, the n_vlong with more than 2048 first)];
 // dims = [n, batch, K, delay, slot0 (spike_buf slot of the first step), n_long, n_vlong, flags (1 Poisson, 4 counts,
 //         8 adaptation)].  Requires 1 <= K <= delay, 2 * delay + K <= 64 (so K <= 21) and 0 <= slot0 < delay.
 FH_EXPORT int launch_lif_window(void** p, const int* dims, uint64_t seed, float a_s, const uint32_t* step_base,

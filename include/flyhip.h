@@ -164,7 +164,7 @@ int launch_pull_bits(void** p, int n, int n_full, int batch, float a_s, int flag
  * step t), so each in-edge's weight and presynaptic word is read once per window; the spikes of every step are also
  * written into the delay ring spike_buf from slot slot0 on.  p = [ptr (n+1), idx, w (float), sched (ceil(n/8)
  * blocks of 8 cells), v, g, refrac, rate, counts, adapt, spikes, drive, pois, active (n), spike_buf (delay, batch,
- * n), hist uint64 (batch, n), P (11), long_rows (n_long: every cell with more than 512 in-edges, most first)];
+ * n), hist uint64 (batch, n), P (11), long_rows (n_long: every cell with more than 1024 in-edges, most first)];
  * dims = [n, batch, K, delay, slot0, n_long, n_vlong (the first n_vlong long cells have more than 2048 in-edges),
  * flags (1 Poisson, 4 counts, 8 adaptation)].  Requires 1 <= K <= delay, 2 * delay + K <= 64 and 0 <= slot0 < delay. */
 int launch_lif_window(void** p, const int* dims, uint64_t seed, float a_s, const uint32_t* step_base, uint32_t step,

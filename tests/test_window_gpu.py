@@ -1,8 +1,9 @@
 """The window stepper of the plain brain (csrc/window.hip, launch_lif_window): up to n_delay steps per launch from
 64-step spike history words, against the CPU oracle's step-by-step reference sequence (decay, ascending-presynaptic
 input of the spikes n_delay steps old, fused LIF update, flyverse/brain.py:830-870), bit for bit, with windows of
-every length from 1 to n_delay that start at every phase of the delay ring, cells of every length class (short,
-long > 512, very long > 2048 and streamed > 9216 in-edges), Poisson forcing, counts and adaptation, and B = 1 or 3."""
+every length from 1 to n_delay that start at every phase of the delay ring, cells of every length class (short up to
+one chunk, short in registers up to 1024 in-edges, long > 1024 on a workgroup, very long > 2048 and streamed > 6144
+in-edges), Poisson forcing, counts and adaptation, and B = 1 or 3."""
 import math
 
 import numpy as np
@@ -21,10 +22,11 @@ CHUNKS = [1, 7, 18, 5, 30, 40, 18, 17, 2, 81]          # 219 steps; windows of e
 
 
 def long_row_graph(seed=0, n=12_000, base_deg=40):
-    """Signed integer-count graph with a Poisson(base_deg) in-degree plus rows of 700, 3,000 and 11,000 in-edges."""
+    """Signed integer-count graph with a Poisson(base_deg) in-degree plus rows of 700, 1,500, 3,000 and 11,000
+    in-edges."""
     rng = np.random.default_rng(seed)
     deg = rng.poisson(base_deg, n)
-    deg[10:20], deg[20:25], deg[25] = 700, min(3000, n - 1), min(11_000, n - 1)
+    deg[10:15], deg[15:20], deg[20:25], deg[25] = 700, 1500, min(3000, n - 1), min(11_000, n - 1)
     post = np.repeat(np.arange(n), deg)
     pre = np.concatenate([rng.choice(n, d, replace=False) for d in deg])
     sign = np.where(rng.random(n) < 0.7, 1.0, -1.0)
@@ -45,7 +47,7 @@ def test_window_stepper_bit_exact_against_the_oracle(B, adapt):
     lp = LIFParams.shiu(dt=0.1, adapt_jump=adapt)
     brain = Brain(c, lp, device=DEV, batch=B, seed=4321)
     assert brain._win_on and brain._win == window_length(brain.n_delay) == 18
-    assert brain._n_long == 16 and brain._n_vlong == 6
+    assert brain._n_long == 11 and brain._n_vlong == 6
     brain.record_activity = True
     rng = np.random.default_rng(1)
     inputs = np.sort(rng.choice(c.n, 400, replace=False))
