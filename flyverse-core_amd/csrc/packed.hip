@@ -92,14 +92,87 @@ __device__ __forceinline__ Window window_of(const Packed& a, int t0, int k) {
     return Window((uint32_t)t0, k, a.delay, a.P[11]);
 }
 
+// Phase A of a block of 64 consecutive cells c0 .. c0 + nc - 1, lanes over each cell's in-edges in turn: the
+// synaptic chain of the window (g_chain_*) of every cell, handed to out(i, gk) (g_k in lane k of gk); it needs
+// nothing of the membrane, so it runs ahead of the LIF update.  With LONG_BUF the chains of the long cells (more than
+// 512 in-edges) were computed before, by k_packed_long / k_packed_vlong: SKIP_LONG leaves them out, otherwise they
+// are read from long_g.
+template <bool LONG_BUF, bool SKIP_LONG, class Out>
+__device__ __forceinline__ void block_chains(const Packed& a, const Window& W, const Edges& E, int c0, int nc,
+                                             int lane, Out out) {
+    const int pl0 = a.ptr[c0 + min(lane, nc)], pl1 = a.ptr[c0 + min(lane + 1, nc)];
+    const float gl = lane < nc ? a.g[c0 + lane] : 0.0f;
+    for (int i = 0; i < nc; ++i) {
+        const int e0 = __builtin_amdgcn_readlane(pl0, i), e1 = __builtin_amdgcn_readlane(pl1, i);
+        const float g0 = fh_readlane(gl, i);
+        float gk;
+        if (e1 - e0 <= kWave) {
+            gk = fh_chain::g_chain_cached<1>(E, W, e0, e1, g0, lane);
+        } else if (e1 - e0 <= 2 * kWave) {
+            gk = fh_chain::g_chain_cached<2>(E, W, e0, e1, g0, lane);
+        } else if (e1 - e0 <= 4 * kWave) {
+            gk = fh_chain::g_chain_cached<4>(E, W, e0, e1, g0, lane);
+        } else if (e1 - e0 <= kShortEdges) {
+            gk = fh_chain::g_chain_cached<8>(E, W, e0, e1, g0, lane);
+        } else {
+            const int li = LONG_BUF && a.n_long > 0 ? a.long_id[c0 + i] : -1;
+            if (li >= 0) {
+                if (SKIP_LONG) continue;
+                gk = lane < W.K ? a.long_g[(int64_t)li * kMaxWindow + lane] : 0.0f;
+            } else {
+                gk = fh_chain::g_chain_stream(E, W, e0, e1, g0, lane);
+            }
+        }
+        out(i, gk);
+    }
+}
+
+// Phase B of the block, lane = cell c0 + lane: the LIF update of flyverse/brain.py:838-868 (Shiu settings) for every
+// step of the window with g_k = gk(k); the lane then rewrites the window's bits of its own cell's history word with
+// one plain store (the block's cells are this wavefront's alone, so no atomics are needed).
+template <class G>
+__device__ __forceinline__ void block_lif(const Packed& a, int T0, int K, int c0, int nc, int lane, G gk) {
+    const float v_rest = a.P[0], v_reset = a.P[1], v_th = a.P[2], am = a.P[3], dt = a.P[4], t_ref = a.P[5];
+    const float ar = a.P[9], rg = a.P[10];
+    const bool ok = lane < nc;
+    const int r = c0 + (ok ? lane : 0);
+    float v = a.v[r], ref = a.refrac[r], rate = a.rate[r], cnt = a.counts[r], g = a.g[r];
+    const float drive = a.drive[r], p = ok ? a.pois[r] : 0.0f;
+    const uint64_t seed = a.seeds[a.inst[r]];
+    const uint32_t loc = (uint32_t)a.local[r];
+    const uint64_t old_hist = a.ring[r];
+    uint64_t mine = 0, fired_bits = 0;    // the window's bit positions, and those of the steps the cell fired
+    for (int k = 0; k < K; ++k) {
+        const int t = T0 + k;
+        const float acc = gk(k);
+        float target = v_rest + acc;
+        target = target + drive;
+        float dv = v - target;
+        dv = dv * am;
+        float vn = target + dv;
+        float rn = ref;
+        if (rn > 0.0f) vn = v_reset;
+        rn = rn - dt;
+        if (rn < 0.0f) rn = 0.0f;
+        float spike = (vn >= v_th) ? 1.0f : 0.0f;
+        if (p > 0.0f && fh_uniform(seed, (uint32_t)t, loc, 0u, 0u) < p) spike = 1.0f;
+        const bool fired = spike > 0.0f;
+        if (fired) { vn = v_reset; rn = t_ref; }
+        float rr = rate * ar;
+        float rs = spike * rg;
+        g = acc; v = vn; ref = rn; rate = rr + rs; cnt = cnt + spike;
+        const uint64_t bit = 1ull << (t & (kSlots - 1));
+        mine |= bit;
+        if (fired) fired_bits |= bit;
+    }
+    if (ok) {
+        a.v[r] = v; a.g[r] = g; a.refrac[r] = ref; a.rate[r] = rate; a.counts[r] = cnt;
+        a.ring[r] = (old_hist & ~mine) | fired_bits;
+    }
+}
+
 // One window: steps T0 .. T0 + K - 1 of every cell.  A wavefront takes a block of 64 consecutive cells (blocks in
-// the order of `sched`) in two phases:
-//  A. for each cell of the block in turn, lanes over its in-edges: the synaptic chain of the window (g_chain_*); it
-//     needs nothing of the membrane, so it runs ahead of the LIF update.  g_k of the cell goes to LDS.  With
-//     LONG_BUF the chains of the long cells (more than 512 in-edges) were computed before by k_packed_long.
-//  B. lanes over the 64 cells: the LIF update of every step of the window with g_k from LDS; each lane then
-//     rewrites the window's bits of its own cell's history word with one plain store (the block's cells are this
-//     wavefront's alone, so no atomics are needed).
+// the order of `sched`): phase A, the chains of its cells into LDS; phase B, their LIF updates.
 template <bool LONG_BUF>
 __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
     __shared__ float gbuf[4][kWave * kMaxWindow];
@@ -107,77 +180,16 @@ __device__ __forceinline__ void packed_window(const Packed& a, int T0, int K) {
     const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = gridDim.x * 4;
     float* const gb = gbuf[threadIdx.x >> 6];
-    const float v_rest = a.P[0], v_reset = a.P[1], v_th = a.P[2], am = a.P[3], dt = a.P[4], t_ref = a.P[5];
-    const float ar = a.P[9], rg = a.P[10];
     const Window W = window_of(a, T0, K);
     const Edges E = edges_of(a);
     for (int bi = wave; bi < a.words; bi += nwaves) {
-        const int wd = a.sched[bi];
-        const int c0 = wd * 64;
+        const int c0 = a.sched[bi] * 64;
         const int nc = min(64, a.n - c0);
-        // the block's row bounds and synaptic states, one lane per cell
-        const int pl0 = a.ptr[c0 + min(lane, nc)], pl1 = a.ptr[c0 + min(lane + 1, nc)];
-        const float gl = lane < nc ? a.g[c0 + lane] : 0.0f;
-        // ---------------------------------------------------------------- phase A
-        for (int i = 0; i < nc; ++i) {
-            const int e0 = __builtin_amdgcn_readlane(pl0, i), e1 = __builtin_amdgcn_readlane(pl1, i);
-            const float g0 = fh_readlane(gl, i);
-            float gk;
-            if (e1 - e0 <= kWave) {
-                gk = fh_chain::g_chain_cached<1>(E, W, e0, e1, g0, lane);
-            } else if (e1 - e0 <= 2 * kWave) {
-                gk = fh_chain::g_chain_cached<2>(E, W, e0, e1, g0, lane);
-            } else if (e1 - e0 <= 4 * kWave) {
-                gk = fh_chain::g_chain_cached<4>(E, W, e0, e1, g0, lane);
-            } else if (e1 - e0 <= kShortEdges) {
-                gk = fh_chain::g_chain_cached<8>(E, W, e0, e1, g0, lane);
-            } else {
-                const int li = LONG_BUF && a.n_long > 0 ? a.long_id[c0 + i] : -1;
-                if (li >= 0)
-                    gk = lane < K ? a.long_g[(int64_t)li * kMaxWindow + lane] : 0.0f;
-                else
-                    gk = fh_chain::g_chain_stream(E, W, e0, e1, g0, lane);
-            }
+        block_chains<LONG_BUF, false>(a, W, E, c0, nc, lane, [&](int i, float gk) {
             if (lane < K) gb[i * kMaxWindow + lane] = gk;
-        }
+        });
         __builtin_amdgcn_wave_barrier();
-        // ---------------------------------------------------------------- phase B
-        // LIF update of flyverse/brain.py:838-868 (Shiu settings), lane = cell c0 + lane
-        const bool ok = lane < nc;
-        const int r = c0 + (ok ? lane : 0);
-        float v = a.v[r], ref = a.refrac[r], rate = a.rate[r], cnt = a.counts[r], g = gl;
-        const float drive = a.drive[r], p = ok ? a.pois[r] : 0.0f;
-        const uint64_t seed = a.seeds[a.inst[r]];
-        const uint32_t loc = (uint32_t)a.local[r];
-        const uint64_t old_hist = a.ring[r];
-        uint64_t mine = 0, fired_bits = 0;    // the window's bit positions, and those of the steps the cell fired
-        for (int k = 0; k < K; ++k) {
-            const int t = T0 + k;
-            const float acc = gb[lane * kMaxWindow + k];
-            float target = v_rest + acc;
-            target = target + drive;
-            float dv = v - target;
-            dv = dv * am;
-            float vn = target + dv;
-            float rn = ref;
-            if (rn > 0.0f) vn = v_reset;
-            rn = rn - dt;
-            if (rn < 0.0f) rn = 0.0f;
-            float spike = (vn >= v_th) ? 1.0f : 0.0f;
-            if (p > 0.0f && fh_uniform(seed, (uint32_t)t, loc, 0u, 0u) < p) spike = 1.0f;
-            const bool fired = spike > 0.0f;
-            if (fired) { vn = v_reset; rn = t_ref; }
-            float rr = rate * ar;
-            float rs = spike * rg;
-            g = acc; v = vn; ref = rn; rate = rr + rs; cnt = cnt + spike;
-            const uint64_t bit = 1ull << (t & (kSlots - 1));
-            mine |= bit;
-            if (fired) fired_bits |= bit;
-        }
-        if (ok) {
-            a.v[r] = v; a.g[r] = g; a.refrac[r] = ref; a.rate[r] = rate; a.counts[r] = cnt;
-            a.ring[r] = (old_hist & ~mine) | fired_bits;
-        }
+        block_lif(a, T0, K, c0, nc, lane, [&](int k) { return gb[lane * kMaxWindow + k]; });
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -223,15 +235,22 @@ __global__ __launch_bounds__(256) void k_packed_long(Packed a, const int* t_base
 // Cells longer than 4 * NCW chunks are streamed by wavefront 0.
 template <int NCW>
 __global__ __launch_bounds__(256) void k_packed_vlong(Packed a, const int* t_base, int off, int K) {
-    __shared__ float carry;
+    __shared__ float lists[4 * (NCW * kWave + kWave)];
+    __shared__ int counts[4];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const Window W = window_of(a, *t_base + off, K);
     const Edges E = edges_of(a);
     for (int i = blockIdx.x; i < a.n_vlong; i += gridDim.x) {
         const int r = a.long_rows[i];
-        const float gk = fh_chain::g_chain_workgroup<NCW>(E, W, a.ptr[r], a.ptr[r + 1], a.g[r], lane, wv, &carry);
+        const int e0 = a.ptr[r], e1 = a.ptr[r + 1];
+        float gk;
+        if (e1 - e0 <= 4 * NCW * kWave)
+            gk = fh_chain::g_chain_compact<NCW>(E, W, e0, e1, a.g[r], lane, wv, lists, counts);
+        else
+            gk = wv == 0 ? fh_chain::g_chain_stream(E, W, e0, e1, a.g[r], lane) : 0.0f;
         if (wv == 0 && lane < K) a.long_g[(int64_t)i * kMaxWindow + lane] = gk;
+        __syncthreads();
     }
 }
 
@@ -299,7 +318,7 @@ FH_EXPORT int launch_packed_run(void** p, const int* dims, void* stream) {
 // whose synaptic chains first kernels compute (the first n_vlong, more than 2048 in-edges, one workgroup per cell;
 // the rest one wavefront per cell; long_id = -1 elsewhere; n_long = 0 streams them inside the window kernel instead);
 // dims = [n, K, window, delay, grid, off, n_long, n_vlong].  The kernel boundaries order the window against the next.
-FH_EXPORT int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream) {
+static int packed_window_parts(void** p, const int* dims, const int* t_base, int parts, void* stream) {
     Packed a;
     fill_packed(a, p);
     a.n = dims[0]; a.steps = dims[1]; a.window = dims[2]; a.delay = dims[3];
@@ -316,16 +335,29 @@ FH_EXPORT int launch_packed_window(void** p, const int* dims, const int* t_base,
         a.long_rows = (const int*)p[18]; a.long_id = (const int*)p[19]; a.long_g = (float*)p[20];
         if (!a.long_rows || !a.long_id || !a.long_g) return FH_ERR_ARGS;
     }
+    if (parts & ~7) return FH_ERR_ARGS;
     if (a.n == 0 || a.steps == 0) return 0;
-    if (a.n_vlong > 0)
+    if ((parts & 1) && a.n_vlong > 0)
         hipLaunchKernelGGL(k_packed_vlong<kVlongChunks>, dim3(min(a.n_vlong, 65535)), dim3(256), 0,
                            (hipStream_t)stream, a, t_base, off, a.steps);
-    if (a.n_long > a.n_vlong) {
+    if ((parts & 2) && a.n_long > a.n_vlong) {
         const int lgrid = min(grid, (a.n_long - a.n_vlong + 3) / 4);
         hipLaunchKernelGGL(k_packed_long, dim3(lgrid), dim3(256), 0, (hipStream_t)stream, a, t_base, off, a.steps);
     }
-    hipLaunchKernelGGL(k_packed_window, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, t_base, off, a.steps);
+    if (parts & 4)
+        hipLaunchKernelGGL(k_packed_window, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, t_base, off, a.steps);
     FH_LAUNCH_CHECK();
+}
+
+FH_EXPORT int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream) {
+    return packed_window_parts(p, dims, t_base, 7, stream);
+}
+
+// Parts of launch_packed_window, same arguments: parts bit 1 = the very long cells' chains, bit 2 = the long cells'
+// chains, bit 4 = the window kernel.  The two chain kernels are independent of each other, so a caller may run them on
+// two streams; the window kernel reads what both wrote and must follow them.
+FH_EXPORT int launch_packed_window_parts(void** p, const int* dims, const int* t_base, int parts, void* stream) {
+    return packed_window_parts(p, dims, t_base, parts, stream);
 }
 
 // *t_base += steps on the stream (the last node of a captured rollout).

@@ -246,6 +246,11 @@ int launch_packed_run(void** p, const int* dims, void* stream);
  * those cells are streamed in the window kernel). */
 int launch_packed_window(void** p, const int* dims, const int* t_base, void* stream);
 
+/* launch_packed_window in parts, same arguments: parts bit 1 = the very long cells' chains, bit 2 = the long cells'
+ * chains, bit 4 = the window kernel (7 = launch_packed_window).  The two chain kernels are independent, so a caller
+ * can run them on two streams; the window kernel reads what both wrote and must follow them. */
+int launch_packed_window_parts(void** p, const int* dims, const int* t_base, int parts, void* stream);
+
 /* *t_base += steps on the stream. */
 int launch_packed_advance(int* t_base, int steps, void* stream);
 
