@@ -334,8 +334,11 @@ struct BackArgs {
     const float* target;                 // (R, 32) targets of step t, or null (t = 0)
     float* gb;                           // (n) per-unit bias gradient accumulator
     float* eg;                           // per-edge gain partials in chunk-padded W^T order, or null
+    float* ly_next;                      // ly of step t - 1 from this step's lv (own rows), or null
+    const float* inp;                    // step t's inputs, for the tau gradient term, or null (no term)
+    float* ga;                           // (n) per-unit tau gradient accumulator
     int rows;
-    float g_ad, a_ad, loss_scale;
+    float g_ad, a_ad, loss_scale, g_rr;
 };
 
 // Backward, part 2 (transpose product + the rest of the step): for unit j
@@ -388,6 +391,19 @@ __global__ __launch_bounds__(256) void k_adj_back_rows(BackArgs k) {
         gbb = gbb + gr;
         gbb = half_sum(gbb);
         if (l == 0) k.gb[j] = k.gb[j] + gbb;
+        // k_adj_back_ly's work for the steps around this one, on the own row: step t's tau term from lv_t+1, and
+        // ly of step t - 1 from lv_t (the same operations, so the same values)
+        if (k.inp) {
+            float dv = k.v[q] - k.inp[q];
+            float cg = lv1 * dv;
+            cg = half_sum(cg);
+            if (l == 0) k.ga[j] = k.ga[j] + cg;
+        }
+        if (k.ly_next) {
+            float oma = 1.0f - k.a[j];
+            float li = lvn * oma;
+            k.ly_next[q] = k.g_rr * li;
+        }
     }
 }
 
@@ -487,7 +503,9 @@ FH_EXPORT int launch_adj_back_ly(void** p, int rows, float g_rr, void* stream) {
 }
 
 // p = [ptrT, idxT, schedT, wT (chunk-padded W^T as launch_adj_forward's W), a, b, ly, v, dr, lv, la, readout,
-// target|null, gb, eg|null (16 per chunk of W^T)]; f = [g_ad, a_ad, loss_scale].
+// target|null, gb, eg|null (16 per chunk of W^T), ly_next|null, inp|null, ga]; f = [g_ad, a_ad, loss_scale, g_rr].
+// With inp the launch also adds step t's tau term to ga (launch_adj_back_ly's ga), and with ly_next it writes ly of
+// step t - 1 from the new lv (launch_adj_back_ly's ly, a buffer other than ly).
 FH_EXPORT int launch_adj_back_rows(void** p, int rows, const float* f, int grid, void* stream) {
     if (rows < 0 || grid <= 0) return FH_ERR_ARGS;
     if (!rows) return 0;
@@ -497,7 +515,9 @@ FH_EXPORT int launch_adj_back_rows(void** p, int rows, const float* f, int grid,
     k.ly = (const float*)p[6]; k.v = (const float*)p[7]; k.dr = (const float*)p[8];
     k.lv = (float*)p[9]; k.la = (float*)p[10]; k.readout = (const int*)p[11]; k.target = (const float*)p[12];
     k.gb = (float*)p[13]; k.eg = (float*)p[14];
-    k.rows = rows; k.g_ad = f[0]; k.a_ad = f[1]; k.loss_scale = f[2];
+    k.ly_next = (float*)p[15]; k.inp = (const float*)p[16]; k.ga = (float*)p[17];
+    k.rows = rows; k.g_ad = f[0]; k.a_ad = f[1]; k.loss_scale = f[2]; k.g_rr = f[3];
+    if ((k.inp && !k.ga) || (k.ly_next && k.ly_next == k.ly)) return FH_ERR_ARGS;
     if (k.eg)
         hipLaunchKernelGGL(k_adj_back_rows<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, k);
     else

@@ -151,6 +151,7 @@ class OpticAdjoint:
         self.S_v, self.S_a, self.S_dr, self.S_inp = self._sets[0]
         self._side = torch.cuda.Stream(device=dev) if nseg > 1 else None
         self.lv, self.la, self.ly = z(n, NL), z(n, NL), z(n, NL)
+        self.ly2 = z(n, NL)                      # ly of the next step, written by the backward launch
         self.ga, self.gb, self.eg = z(n), z(n), z(len(bidx))
         self._params = None
         self._last_segment_ready = False
@@ -242,7 +243,7 @@ class OpticAdjoint:
             raise RuntimeError("call loss() first")
         n, C, T = self.n, self.checkpoint, self.T
         scale = 2.0 / (T * NL * self.R)
-        f_back = (ct.c_float * 3)(self.c.adapt_gain, self.a_ad, scale)
+        f_back = (ct.c_float * 4)(self.c.adapt_gain, self.a_ad, scale, self.c.gain_rr)
         for buf in (self.lv, self.la, self.ga, self.gb, self.eg):
             buf.zero_()
         nseg = (T + C - 1) // C
@@ -250,6 +251,7 @@ class OpticAdjoint:
         if self._side is not None:
             self._side.wait_stream(main)         # the forward's checkpoints and parameters first
         freed = [None, None]                     # per buffer set: the event after the last backward that read it
+        lys = [self.ly, self.ly2]                # ly of the current step, and of the next one
         ready = {}                               # per segment: the event after its recompute
 
         def recompute(s, which):
@@ -281,14 +283,16 @@ class OpticAdjoint:
             S_v, S_a, S_dr, S_inp = self._sets[which]
             t0, t1 = s * C, min(T, (s + 1) * C)
             if s == last:
-                # the loss term of the last step, T: lambda_v_T = loss_scale (r_T - target_T) gate_T
-                self.ly.zero_()
-                self._back_rows(t1 - t0, T, f_back, None, S_v, S_dr)
+                # the loss term of the last step, T: lambda_v_T = loss_scale (r_T - target_T) gate_T; it also
+                # writes ly of step T - 1
+                lys[0].zero_()
+                self._back_rows(t1 - t0, T, f_back, None, S_v, S_dr, lys[0], lys[1])
+                lys.reverse()
             for k in reversed(range(t1 - t0)):
                 t = t0 + k
-                self._call("launch_adj_back_ly", [self.lv, S_v[k], S_inp[k], self.a, self.ly, self.ga], n,
-                           float(self.c.gain_rr))
-                self._back_rows(k, t, f_back, self.eg, S_v, S_dr)
+                # step t from ly_t: also step t's tau term and ly of step t - 1 (one launch per step)
+                self._back_rows(k, t, f_back, self.eg, S_v, S_dr, lys[0], lys[1], S_inp[k])
+                lys.reverse()
             ev = torch.cuda.Event()
             ev.record(main)
             freed[which] = ev
@@ -303,16 +307,18 @@ class OpticAdjoint:
         g_tau = ga_t * a_k * (self.c.dt / (tau * tau))
         return g_tau, gb_t, gg.view(K, K)
 
-    def _back_rows(self, k, t, f_back, eg=None, S_v=None, S_dr=None):
+    def _back_rows(self, k, t, f_back, eg=None, S_v=None, S_dr=None, ly=None, ly_next=None, inp=None):
         """The transposed product and the rest of step t's adjoint (states of step t: row k of the segment buffers
-        S_v, S_dr, by default the first set); with eg, the same gathers also add each edge's sum over movies of
-        ly[row] * dr_t[col] to eg (chunk-padded W^T order)."""
+        S_v, S_dr, by default the first set; ly of step t, by default self.ly); with eg, the same gathers also add
+        each edge's sum over movies of ly[row] * dr_t[col] to eg (chunk-padded W^T order); with inp (step t's
+        inputs) step t's tau term goes to ga, and with ly_next ly of step t - 1 is written there."""
         S_v = self.S_v if S_v is None else S_v
         S_dr = self.S_dr if S_dr is None else S_dr
+        ly = self.ly if ly is None else ly
         target = self.target[t - 1] if t >= 1 else None
         self._call("launch_adj_back_rows",
-                   [self.bptr, self.bidx, self.schedT, self.bw, self.a, self.b, self.ly, S_v[k], S_dr[k],
-                    self.lv, self.la, self.readout, target, self.gb, eg],
+                   [self.bptr, self.bidx, self.schedT, self.bw, self.a, self.b, ly, S_v[k], S_dr[k],
+                    self.lv, self.la, self.readout, target, self.gb, eg, ly_next, inp, self.ga],
                    self.n, f_back, self.grid)
 
 

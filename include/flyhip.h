@@ -281,8 +281,11 @@ int launch_adj_forward(void** p, int rows, const float* f, int grid, void* strea
 int launch_adj_back_ly(void** p, int rows, float g_rr, void* stream);
 
 /* Adjoint of one step through the transpose CSR (W^T ly) and the clip gates.  p = [ptrT, idxT, schedT, wT, a, b, ly,
- * v, dr, lv, la, readout, target|null, gb, eg|null]; f = [g_ad, a_ad, loss_scale].  With eg (16 floats per chunk of
- * the chunk-padded W^T) each real entry also gets eg += sum over movies of ly[row] * dr[col], from the same gathers. */
+ * v, dr, lv, la, readout, target|null, gb, eg|null, ly_next|null, inp|null, ga|null]; f = [g_ad, a_ad, loss_scale,
+ * g_rr].  With eg (16 floats per chunk of the chunk-padded W^T) each real entry also gets eg += sum over movies of
+ * ly[row] * dr[col], from the same gathers.  With inp (step t's inputs) the launch also does launch_adj_back_ly's ga
+ * term for step t, and with ly_next (a buffer other than ly) launch_adj_back_ly's ly for step t - 1 from the new lv,
+ * so a backward step needs this launch alone. */
 int launch_adj_back_rows(void** p, int rows, const float* f, int grid, void* stream);
 
 /* out[s] = fixed-order sum over segment s of vals[perm[j]] (* mult[perm[j]]).  p = [vals, mult|null, perm,

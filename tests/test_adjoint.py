@@ -130,7 +130,7 @@ def test_adjoint_edge_partials_from_the_transposed_product():
     prob.S_dr[0].copy_(t(dr))
     eg0 = rng.normal(0, 1, prob.eg.numel()).astype(np.float32)
     eg = t(eg0.copy())
-    f_back = (ct.c_float * 3)(1.0, prob.a_ad, 0.01)
+    f_back = (ct.c_float * 4)(1.0, prob.a_ad, 0.01, 1.0)
     prob._back_rows(0, 1, f_back, eg)
     torch.cuda.synchronize()
     got = eg.cpu().numpy()
