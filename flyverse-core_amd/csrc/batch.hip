@@ -276,16 +276,15 @@ struct BatchLif {
     const float* P;          // (12) v_rest, v_reset, v_th, a_m, dt, t_ref, -, -, -, a_r, rate_gain, a_s
     const uint8_t* act;      // mode 6: (n_rows) bit k = the mask step k of the launch reads is nonzero
     const uint64_t* zero;    // mode 6: one zero word, the address of every gather whose mask is known to be zero
-    const uint64_t* win;     // mode 7: (n_rows + 1, 8) the K masks of the launch per row, contiguous; last row zero
     int n_rows, lanes, ring_slots, delay, record, mode, row_major;
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 7 ? 4 : 1))) void k_batch_lif(BatchLif a, int step_offset, int K) {
+__global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, int K) {
     constexpr bool L5 = MODE >= 5;                  // LDS-staged weights
-    constexpr bool ACT = MODE >= 6;                 // gathers skipped where the activity bits say the mask is zero
+    constexpr bool ACT = MODE == 6;                 // gathers skipped where the activity bits say the mask is zero
     __shared__ float wstage[MODE == 2 ? 4 : 1][64];
     __shared__ uint64_t cmask[MODE == 4 ? 4 : 1][256];
     __shared__ float cweight[MODE == 4 ? 4 : 1][256];
@@ -477,57 +476,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 7 ?
                         }
                     }
                 };
-                if (MODE == 7 && G == 1) {
-                    // Rows of at most 4 chunks: one window gather per edge for the whole launch (its presynaptic
-                    // cell's K <= 6 masks, 48 contiguous bytes of win, or the zero row when the activity byte says
-                    // they are all zero), each step's 64 x 64 transpose kept in registers, then the K folds and LIF
-                    // updates.  The same adds in the same order as mode 5.
-                    const int nch = (ne + 63) >> 6;
-                    uint32_t oc[4];
-                    float wc[4];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const int e = min(e0 + c * 64 + lane, e1 - 1);
-                        oc[c] = a.pre_off[e];
-                        wc[c] = a.w[e];
-                    }
-                    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) ws[c * 66 + ws_lane] = wc[c];
-                    __builtin_amdgcn_wave_barrier();
-                    uint32_t lo[4][6], hi[4][6];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        if (c < nch) {
-                            const uint32_t pre = oc[c] >> 3;
-                            const bool on = c * 64 + lane < ne && a.act[pre] != 0;
-                            const uint4* src = (const uint4*)(a.win + (size_t)(on ? pre : (uint32_t)a.n_rows) * 8);
-                            const uint4 q0 = src[0], q1 = src[1], q2 = src[2];
-                            const uint64_t m[6] = {q0.x | ((uint64_t)q0.y << 32), q0.z | ((uint64_t)q0.w << 32),
-                                                   q1.x | ((uint64_t)q1.y << 32), q1.z | ((uint64_t)q1.w << 32),
-                                                   q2.x | ((uint64_t)q2.y << 32), q2.z | ((uint64_t)q2.w << 32)};
-#pragma unroll
-                            for (int k = 0; k < 6; ++k)
-                                if (k < K) transpose_chunk_x(m[k], lane, lo[c][k], hi[c][k]);
-                        }
-                    }
-                    uint32_t ws_slot = w0;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) {
-                        if (k < K) {
-                            float acc = g * a_s;
-#pragma unroll
-                            for (int c = 0; c < 4; ++c) {
-                                if (c < nch) {
-                                    acc = add_bits_lds(acc, lo[c][k], ws + 1 + (2 * c) * 33);
-                                    acc = add_bits_lds(acc, hi[c][k], ws + 1 + (2 * c + 1) * 33);
-                                }
-                            }
-                            lif_step(k, acc, ws_slot);
-                            ws_slot = ws_slot + 1 == R ? 0 : ws_slot + 1;
-                        }
-                    }
-                } else if (G <= 2) {
+                if (G <= 2) {
                     // offsets and weights of the row's (at most 8) chunks stay in registers for the whole launch
                     uint32_t oc[8];
                     float wc[8];
@@ -986,31 +935,6 @@ __global__ __launch_bounds__(256) void k_batch_act(const uint64_t* __restrict__ 
     act[r] = (uint8_t)bits;
 }
 
-// Fold mode 7's window: win[r * 8 + k] = the mask step k of the launch reads (k < K, 0 beyond), and the activity
-// byte of mode 6.  Row n_rows of win stays zero (the address of a gather whose masks are all zero).
-__global__ __launch_bounds__(256) void k_batch_win(const uint64_t* __restrict__ ring, const uint32_t* engine_step,
-                                                   int step_offset, int K, int n_rows, int R, int D,
-                                                   uint8_t* __restrict__ act, uint64_t* __restrict__ win) {
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= n_rows) return;
-    const uint32_t T0 = *engine_step + (uint32_t)step_offset;
-    uint32_t slot = (T0 + (uint32_t)R - (uint32_t)D) % (uint32_t)R;
-    uint32_t bits = 0u;
-    uint64_t m[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        m[k] = k < K ? ring[(size_t)slot * n_rows + r] : 0ull;
-        if (m[k] != 0ull) bits |= 1u << k;
-        slot = slot + 1 == (uint32_t)R ? 0u : slot + 1;
-    }
-    uint4* dst = (uint4*)(win + (size_t)r * 8);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        dst[k] = make_uint4((uint32_t)m[2 * k], (uint32_t)(m[2 * k] >> 32), (uint32_t)m[2 * k + 1],
-                            (uint32_t)(m[2 * k + 1] >> 32));
-    act[r] = (uint8_t)bits;
-}
-
 }  // namespace
 
 // Up to `delay` LIF steps of every row for 64 flies.  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v, g,
@@ -1037,31 +961,19 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     a.mode = dims[6];
     a.row_major = dims[7];
     if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
-        grid <= 0 || a.mode < 0 || a.mode > 7)
+        grid <= 0 || a.mode < 0 || a.mode > 6)
         return FH_ERR_ARGS;
     a.act = nullptr;
     a.zero = nullptr;
-    a.win = nullptr;
-    if (a.mode >= 6) {
-        if (a.row_major || K > (a.mode == 7 ? 6 : 8) || p[19] == nullptr) return FH_ERR_ARGS;
+    if (a.mode == 6) {
+        if (a.row_major || K > 8 || p[19] == nullptr) return FH_ERR_ARGS;
         a.act = (const uint8_t*)p[19];
         a.zero = (const uint64_t*)(a.act + (((size_t)a.n_rows + 7) & ~(size_t)7));
-    }
-    if (a.mode == 7) {
-        if (p[20] == nullptr) return FH_ERR_ARGS;
-        a.win = (const uint64_t*)p[20];
     }
     if (a.n_rows == 0 || K == 0) return 0;
     if (a.mode == 6) {
         hipLaunchKernelGGL(k_batch_act, dim3(nblk(a.n_rows)), dim3(256), 0, (hipStream_t)stream, a.ring,
                            a.engine_step, step_offset, K, a.n_rows, a.ring_slots, a.delay, (uint8_t*)p[19]);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return (int)e;
-    }
-    if (a.mode == 7) {
-        hipLaunchKernelGGL(k_batch_win, dim3(nblk(a.n_rows)), dim3(256), 0, (hipStream_t)stream, a.ring,
-                           a.engine_step, step_offset, K, a.n_rows, a.ring_slots, a.delay, (uint8_t*)p[19],
-                           (uint64_t*)p[20]);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
     }
@@ -1072,8 +984,7 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
         case 3: hipLaunchKernelGGL(k_batch_lif<3>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 4: hipLaunchKernelGGL(k_batch_lif<4>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 5: hipLaunchKernelGGL(k_batch_lif<5>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
-        case 6: hipLaunchKernelGGL(k_batch_lif<6>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
-        default: hipLaunchKernelGGL(k_batch_lif<7>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
+        default: hipLaunchKernelGGL(k_batch_lif<6>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
     }
     FH_LAUNCH_CHECK();
 }
