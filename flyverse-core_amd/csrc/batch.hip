@@ -276,29 +276,21 @@ struct BatchLif {
     const float* P;          // (12) v_rest, v_reset, v_th, a_m, dt, t_ref, -, -, -, a_r, rate_gain, a_s
     const uint8_t* act;      // mode 6: (n_rows) bit k = the mask step k of the launch reads is nonzero
     const uint64_t* zero;    // mode 6: one zero word, the address of every gather whose mask is known to be zero
-    const uint64_t* win;     // mode 7: (n_rows + 1, 8) the K masks of the launch per row, contiguous; last row zero
     int n_rows, lanes, ring_slots, delay, record, mode, row_major;
-    int win_reg, win_lds;    // mode 7: rows of at most win_reg chunks hold their windows in registers, of at most
-                             // win_lds chunks in LDS; longer rows gather per step as in mode 6
 };
-
-typedef __attribute__((address_space(3))) void* lds_ptr;
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, int K) {
     constexpr bool L5 = MODE >= 5;                  // LDS-staged weights
-    constexpr bool ACT = MODE >= 6;                 // gathers skipped where the activity bits say the mask is zero
+    constexpr bool ACT = MODE == 6;                 // gathers skipped where the activity bits say the mask is zero
     __shared__ float wstage[MODE == 2 ? 4 : 1][64];
     __shared__ uint64_t cmask[MODE == 4 ? 4 : 1][256];
     __shared__ float cweight[MODE == 4 ? 4 : 1][256];
     // mode 5: per wavefront 16 segments of 33 floats, [-0.0, the weights of 32 consecutive edges]; segments 2c and
     // 2c + 1 hold the two halves of chunk c of the row (rows of up to 512 edges) or of the current work item
     __shared__ float wseg[L5 ? 4 : 1][L5 ? 16 * 33 : 1];
-    // mode 7: per wavefront 14 chunk-steps of masks (7 KB), filled by direct-to-LDS loads
-    __shared__ uint64_t wwin[MODE == 7 ? 4 : 1][MODE == 7 ? 14 * 64 : 1];
-    uint64_t* const wwin_w = wwin[MODE == 7 ? threadIdx.x >> 6 : 0];
     const int lane = threadIdx.x & 63;
     float* const ws = wseg[L5 ? threadIdx.x >> 6 : 0];
     const int ws_lane = (lane >> 5) * 33 + 1 + (lane & 31);      // this lane's weight slot within a chunk's segments
@@ -367,7 +359,6 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                 for (int k = 0; k < K; ++k) lif_step(k, g * a_s, (T0 + (uint32_t)k) % R);
             } else {
                 const int G = (ne + 255) >> 8;
-                const int nch_w = (ne + 63) >> 6;
                 // a cursor over the work items (step k, group grp), groups inner, with the ring slots of step k
                 // (read: step t - D, write: step t) advanced by increments instead of divisions
                 struct Cursor { int k, grp; uint32_t rslot, wslot; };
@@ -485,116 +476,7 @@ __global__ __launch_bounds__(256) void k_batch_lif(BatchLif a, int step_offset, 
                         }
                     }
                 };
-                if (MODE == 7 && nch_w <= a.win_reg) {
-                    // Rows of at most 2 chunks: each lane loads its edge's window (48 bytes, three 16-byte loads
-                    // of one cache line) into registers once per launch; step k transposes mask k.
-                    const int nch = nch_w;
-                    uint32_t srow[2];
-                    float wc[2];
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        const int e = e0 + c * 64 + lane;
-                        const bool in = e < e1;
-                        srow[c] = in ? a.pre_off[e] >> 3 : (uint32_t)a.n_rows;
-                        wc[c] = in ? a.w[e] : 0.0f;
-                    }
-                    uint64_t m[2][6];
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        const uint4* src = (const uint4*)(a.win + (size_t)(c < nch ? srow[c] : (uint32_t)a.n_rows) * 8);
-                        const uint4 q0 = src[0], q1 = src[1], q2 = src[2];
-                        m[c][0] = q0.x | ((uint64_t)q0.y << 32); m[c][1] = q0.z | ((uint64_t)q0.w << 32);
-                        m[c][2] = q1.x | ((uint64_t)q1.y << 32); m[c][3] = q1.z | ((uint64_t)q1.w << 32);
-                        m[c][4] = q2.x | ((uint64_t)q2.y << 32); m[c][5] = q2.z | ((uint64_t)q2.w << 32);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) ws[c * 66 + ws_lane] = wc[c];
-                    __builtin_amdgcn_wave_barrier();
-                    uint32_t ws_slot = w0;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) {
-                        if (k < K) {
-                            float acc = g * a_s;
-#pragma unroll
-                            for (int c = 0; c < 2; ++c) {
-                                if (c < nch) {
-                                    uint32_t lo, hi;
-                                    transpose_chunk_x(m[c][k], lane, lo, hi);
-                                    acc = add_bits_lds(acc, lo, ws + 1 + (2 * c) * 33);
-                                    acc = add_bits_lds(acc, hi, ws + 1 + (2 * c + 1) * 33);
-                                }
-                            }
-                            lif_step(k, acc, ws_slot);
-                            ws_slot = ws_slot + 1 == R ? 0 : ws_slot + 1;
-                        }
-                    }
-                } else if (MODE == 7 && nch_w <= a.win_lds) {
-                    // Rows of at most 7 chunks read their masks from the launch window win (row p's K masks in 64
-                    // contiguous bytes), a sub-window of S steps at a time, straight into this wavefront's LDS area:
-                    // S = 6 steps for rows of 1-2 chunks, 4 for 3 chunks, 2 for 4-7 (at most 14 chunk-steps, 7 KB).
-                    // One 16-byte load carries two steps of one edge, and the U = S / 2 loads of an edge are
-                    // adjacent lanes of one instruction, so each edge costs one cache line per sub-window instead of
-                    // one per step.  Lanes past the row's end read the zero row.  The adds are mode 5's, in the same
-                    // order.
-                    const int nch = nch_w;
-                    uint32_t srow[7];
-                    float wc[7];
-#pragma unroll
-                    for (int c = 0; c < 7; ++c) {
-                        const int e = e0 + c * 64 + lane;
-                        const bool in = e < e1;
-                        srow[c] = in ? a.pre_off[e] >> 3 : (uint32_t)a.n_rows;
-                        wc[c] = in ? a.w[e] : 0.0f;
-                    }
-                    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                    for (int c = 0; c < 7; ++c) ws[c * 66 + ws_lane] = wc[c];
-                    __builtin_amdgcn_wave_barrier();
-                    const int S = nch <= 2 ? 6 : nch == 3 ? 4 : 2;
-                    uint32_t ws_slot = w0;
-                    for (int k0 = 0; k0 < K; k0 += S) {
-                        const int Sk = min(S, K - k0);
-                        const int U = (Sk + 1) >> 1;             // 16-byte loads per edge
-                        // the previous sub-window's LDS reads have returned before the loads overwrite the area
-                        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-#pragma unroll
-                        for (int c = 0; c < 7; ++c) {
-                            if (c < nch) {
-#pragma unroll
-                                for (int i = 0; i < 3; ++i) {
-                                    if (i < U) {
-                                        // unit u = i * 64 + lane is part u % U of edge u / U (u < 192)
-                                        const uint32_t u = (uint32_t)(i * 64 + lane);
-                                        const uint32_t e = U == 1 ? u : U == 2 ? u >> 1 : (u * 43691u) >> 17;
-                                        const uint32_t part = u - e * (uint32_t)U;
-                                        const uint32_t sr = U == 1 ? srow[c]
-                                            : (uint32_t)__builtin_amdgcn_ds_bpermute((int)(e << 2), (int)srow[c]);
-                                        const uint64_t* src = a.win + (size_t)sr * 8 + k0 + 2 * part;
-                                        __builtin_amdgcn_global_load_lds((void*)src,
-                                            (lds_ptr)(wwin_w + (c * U + i) * 128), 16, 0, 0);
-                                    }
-                                }
-                            }
-                        }
-                        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                        for (int kk = 0; kk < Sk; ++kk) {
-                            float acc = g * a_s;
-#pragma unroll
-                            for (int c = 0; c < 7; ++c) {
-                                if (c < nch) {
-                                    const uint64_t m = wwin_w[c * U * 128 + lane * 2 * U + kk];
-                                    uint32_t lo, hi;
-                                    transpose_chunk_x(m, lane, lo, hi);
-                                    acc = add_bits_lds(acc, lo, ws + 1 + (2 * c) * 33);
-                                    acc = add_bits_lds(acc, hi, ws + 1 + (2 * c + 1) * 33);
-                                }
-                            }
-                            lif_step(k0 + kk, acc, ws_slot);
-                            ws_slot = ws_slot + 1 == R ? 0 : ws_slot + 1;
-                        }
-                    }
-                } else if (G <= 2) {
+                if (G <= 2) {
                     // offsets and weights of the row's (at most 8) chunks stay in registers for the whole launch
                     uint32_t oc[8];
                     float wc[8];
@@ -1053,31 +935,6 @@ __global__ __launch_bounds__(256) void k_batch_act(const uint64_t* __restrict__ 
     act[r] = (uint8_t)bits;
 }
 
-// Fold mode 7's window: win[r * 8 + k] = the mask step k of the launch reads (k < K, 0 beyond), and the activity
-// byte of mode 6.  Row n_rows of win stays zero (the address of a gather whose masks are all zero).
-__global__ __launch_bounds__(256) void k_batch_win(const uint64_t* __restrict__ ring, const uint32_t* engine_step,
-                                                   int step_offset, int K, int n_rows, int R, int D,
-                                                   uint8_t* __restrict__ act, uint64_t* __restrict__ win) {
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= n_rows) return;
-    const uint32_t T0 = *engine_step + (uint32_t)step_offset;
-    uint32_t slot = (T0 + (uint32_t)R - (uint32_t)D) % (uint32_t)R;
-    uint32_t bits = 0u;
-    uint64_t m[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        m[k] = k < K ? ring[(size_t)slot * n_rows + r] : 0ull;
-        if (m[k] != 0ull) bits |= 1u << k;
-        slot = slot + 1 == (uint32_t)R ? 0u : slot + 1;
-    }
-    uint4* dst = (uint4*)(win + (size_t)r * 8);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        dst[k] = make_uint4((uint32_t)m[2 * k], (uint32_t)(m[2 * k] >> 32), (uint32_t)m[2 * k + 1],
-                            (uint32_t)(m[2 * k + 1] >> 32));
-    act[r] = (uint8_t)bits;
-}
-
 }  // namespace
 
 // Up to `delay` LIF steps of every row for 64 flies.  p = [ptr, pre_off, w, sched, neuron_id, row_flags, v, g,
@@ -1104,37 +961,19 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
     a.mode = dims[6];
     a.row_major = dims[7];
     if (a.n_rows < 0 || a.lanes < 0 || a.lanes > 64 || K < 0 || K > a.delay || a.ring_slots < 2 * a.delay ||
-        grid <= 0 || a.mode < 0 || a.mode > 7)
+        grid <= 0 || a.mode < 0 || a.mode > 6)
         return FH_ERR_ARGS;
     a.act = nullptr;
     a.zero = nullptr;
-    a.win = nullptr;
-    if (a.mode >= 6) {
-        if (a.row_major || K > (a.mode == 7 ? 6 : 8) || p[19] == nullptr) return FH_ERR_ARGS;
+    if (a.mode == 6) {
+        if (a.row_major || K > 8 || p[19] == nullptr) return FH_ERR_ARGS;
         a.act = (const uint8_t*)p[19];
         a.zero = (const uint64_t*)(a.act + (((size_t)a.n_rows + 7) & ~(size_t)7));
-    }
-    a.win_reg = 0;
-    a.win_lds = 0;
-    if (a.mode == 7) {
-        if (p[20] == nullptr) return FH_ERR_ARGS;
-        a.win = (const uint64_t*)p[20];
-        const char* wr = getenv("FH_WIN_REG");
-        const char* wl = getenv("FH_WIN_LDS");
-        a.win_reg = wr ? min(2, atoi(wr)) : 2;
-        a.win_lds = wl ? min(7, atoi(wl)) : 7;
     }
     if (a.n_rows == 0 || K == 0) return 0;
     if (a.mode == 6) {
         hipLaunchKernelGGL(k_batch_act, dim3(nblk(a.n_rows)), dim3(256), 0, (hipStream_t)stream, a.ring,
                            a.engine_step, step_offset, K, a.n_rows, a.ring_slots, a.delay, (uint8_t*)p[19]);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return (int)e;
-    }
-    if (a.mode == 7) {
-        hipLaunchKernelGGL(k_batch_win, dim3(nblk(a.n_rows)), dim3(256), 0, (hipStream_t)stream, a.ring,
-                           a.engine_step, step_offset, K, a.n_rows, a.ring_slots, a.delay, (uint8_t*)p[19],
-                           (uint64_t*)p[20]);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
     }
@@ -1145,8 +984,7 @@ FH_EXPORT int launch_batch_lif(void** p, const int* dims, int step_offset, int K
         case 3: hipLaunchKernelGGL(k_batch_lif<3>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 4: hipLaunchKernelGGL(k_batch_lif<4>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
         case 5: hipLaunchKernelGGL(k_batch_lif<5>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
-        case 6: hipLaunchKernelGGL(k_batch_lif<6>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K); break;
-        default: hipLaunchKernelGGL(k_batch_lif<7>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
+        default: hipLaunchKernelGGL(k_batch_lif<6>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, step_offset, K);
     }
     FH_LAUNCH_CHECK();
 }
