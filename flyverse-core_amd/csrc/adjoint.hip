@@ -31,31 +31,45 @@ __device__ __forceinline__ float half_sum(float x) {
     return x;
 }
 
+// DPP lane moves (gfx9 encodings): lane i of a 16-lane row reads lane i + 4 (row_shl:4), i - 4 (row_shr:4) or
+// (i + 8) mod 16 = i ^ 8 (row_ror:8); within a quad, lane i ^ 1 or i ^ 2 (quad_perm).  BANKS selects the groups of
+// four lanes of each row that are written; the others keep `old`.
+constexpr int kDppShl4 = 0x104, kDppShr4 = 0x114, kDppRor8 = 0x128, kDppXor1 = 0xB1, kDppXor2 = 0x4E;
+
+template <int CTRL, int BANKS = 0xf>
+__device__ __forceinline__ float dpp_move(float old, float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(x), CTRL, 0xf, BANKS,
+                                                      false));
+}
+
 // The 16 sums over the 32 lanes of a half-wave of v[0..15] at once, by a transposing butterfly: at each of four
-// exchange steps a lane keeps one half of its values and adds its partner's copy of them, so the values per lane
-// halve; a last exchange completes the sums.  Lanes 2m and 2m + 1 of the half end with the sum of v[m] (the same
-// fixed tree of additions for every m), for 16 shuffles and 16 adds in all.
+// exchange steps (partners l ^ 16, l ^ 8, l ^ 4, l ^ 2) a lane keeps one half of its values and adds its partner's
+// copy of them, so the values per lane halve; a last exchange (l ^ 1) completes the sums.  Lanes 2m and 2m + 1 of
+// the half end with the sum of v[m] (the same fixed tree of additions for every m).  Every exchange is a VALU lane
+// move: one v_permlane16_swap per pair at the first step (lane l < 16 of the half gets lane l + 16's copy of the
+// value it keeps and gives its own copy of the other, so the swap itself does the keep/give choice), DPP after it.
 __device__ __forceinline__ float half_sums16(const float (&v)[16], int l) {
     float a8[8], a4[4], a2[2];
-    const bool u16 = l & 16, u8 = l & 8, u4 = l & 4, u2 = l & 2;
+    const bool u8 = l & 8, u4 = l & 4, u2 = l & 2;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const float keep = u16 ? v[k + 8] : v[k], give = u16 ? v[k] : v[k + 8];
-        a8[k] = keep + __shfl_xor(give, 16, 32);
+        const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[k]), __float_as_uint(v[k + 8]), false,
+                                                         false);
+        a8[k] = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const float keep = u8 ? a8[k + 4] : a8[k], give = u8 ? a8[k] : a8[k + 4];
-        a4[k] = keep + __shfl_xor(give, 8, 32);
+        a4[k] = keep + dpp_move<kDppRor8>(0.0f, give);
     }
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const float keep = u4 ? a4[k + 2] : a4[k], give = u4 ? a4[k] : a4[k + 2];
-        a2[k] = keep + __shfl_xor(give, 4, 32);
+        a2[k] = keep + dpp_move<kDppShl4, 0x5>(dpp_move<kDppShr4, 0xa>(0.0f, give), give);
     }
     const float keep = u2 ? a2[1] : a2[0], give = u2 ? a2[0] : a2[1];
-    float a1 = keep + __shfl_xor(give, 2, 32);
-    a1 = a1 + __shfl_xor(a1, 1, 32);
+    float a1 = keep + dpp_move<kDppXor2>(0.0f, give);
+    a1 = a1 + dpp_move<kDppXor1>(0.0f, a1);
     return a1;
 }
 
@@ -141,9 +155,12 @@ __device__ __forceinline__ void unroll16(F&& f) {
 //
 // With EG, each chunk also adds to eg[16 c + m] (the chunk-padded order of this CSR) the sum over the half's 32 lanes
 // of x[i_m, l] * own[l], own being the half's own row of another (rows, 32) array: the per-edge products of the
-// transposed product's gathers with the row's own values (half_sums16), written once per real entry.
+// transposed product's gathers with the row's own values (half_sums16), written once per real entry.  The old
+// eg[16 c + m] is loaded with the chunk's gathers, so the fold's add waits for that stage only: a load issued in
+// the fold would wait for every younger stage's loads too (vector loads complete in issue order).
 struct HalfStage {
     float x[16];
+    float eg;                            // EG: eg[16 c + m] of this chunk, m = (l >> 1) & 15
     uint32_t real;                       // bit j: entry j of this half's chunk is real (not a pad)
 };
 
@@ -160,7 +177,9 @@ __device__ __forceinline__ float fold32_pair_padded(const int* __restrict__ pidx
                                      : (const uint32_t*)pw + (size_t)c * 16 + (l - 16);
         return *src;
     };
-    auto issue = [&](uint32_t V, HalfStage& st) {
+    const int m = (l >> 1) & 15;                  // the entry whose gain partial this lane pair writes
+    // the gathers of chunk k (clamped to the half's last chunk, as load_v), and with EG its old partials
+    auto issue = [&](int k, uint32_t V, HalfStage& st) {
         st.real = 0;
         unroll16<0>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
@@ -168,6 +187,7 @@ __device__ __forceinline__ float fold32_pair_padded(const int* __restrict__ pidx
             st.real |= (ij >= 0 ? 1u : 0u) << j;
             st.x[j] = x[(size_t)max(ij, 0) * NL + l];
         });
+        if (EG) st.eg = eg[(int64_t)max(c0 + min(k, nh - 1), 0) * 16 + m];
     };
     // the barriers keep the issue order load_v, gathers, fold (the waits depend on it)
     auto pin = [] { __asm__ volatile("" ::: "memory"); __builtin_amdgcn_sched_barrier(0); };
@@ -182,15 +202,12 @@ __device__ __forceinline__ float fold32_pair_padded(const int* __restrict__ pidx
         });
         if (k < nh) total = total + s;
         if (EG) {
-            // the products with the row's own values, in place of the gathered values (pads contribute 0)
+            // the products with the row's own values, in place of the gathered values; a pad's product (row 0's
+            // values, finite) only enters the sum of its own entry, which is not written
 #pragma unroll
-            for (int j = 0; j < 16; ++j) b.x[j] = ((b.real >> j) & 1u) ? b.x[j] * own : 0.0f;
+            for (int j = 0; j < 16; ++j) b.x[j] = b.x[j] * own;
             const float e = half_sums16(b.x, l);
-            const int m = (l >> 1) & 15;
-            if (k < nh && (l & 1) == 0 && ((b.real >> m) & 1u)) {
-                float* q = eg + (int64_t)(c0 + k) * 16 + m;
-                *q = *q + e;
-            }
+            if (k < nh && (l & 1) == 0 && ((b.real >> m) & 1u)) eg[(int64_t)(c0 + k) * 16 + m] = b.eg + e;
         }
     };
     // iteration k: the word of chunk k + 2 is loaded, chunk k + 1 gathered, chunk k folded.  Words rotate through
@@ -198,12 +215,12 @@ __device__ __forceinline__ float fold32_pair_padded(const int* __restrict__ pidx
     HalfStage S0, S1;
     uint32_t V0 = load_v(0), V1 = load_v(1), V2;
     pin();
-    issue(V0, S0);
+    issue(0, V0, S0);
     pin();
     auto step = [&](int k, uint32_t& vload, uint32_t vissue, HalfStage& sissue, HalfStage& sfold, uint32_t vfold) {
         vload = load_v(k + 2);
         pin();
-        issue(vissue, sissue);
+        issue(k + 1, vissue, sissue);
         pin();
         fold(k, sfold, vfold);
     };
