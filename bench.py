@@ -147,7 +147,7 @@ def spmv_roofline(conn, retina, op, reps=50):
     from flyhip.optic import optic_matrices
     M = optic_matrices(conn, retina, op)["M_rr"]
     out = {}
-    for kernel, rc in (("lds", 0), ("grouped", 0), ("chunked", 0)):
+    for kernel, rc in (("panel", 0), ("lds", 0), ("grouped", 0), ("chunked", 0)):
         csr = hip.CSR(M, "cuda", kernel=kernel, range_cols=rc)
         kernel = kernel + (f"{rc}" if rc else "")
         x = torch.rand(1, M.shape[1], device="cuda")

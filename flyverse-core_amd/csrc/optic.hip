@@ -3,6 +3,7 @@
 // photoreceptor stage optic.py:496-517, the feedback optic.py:572 and the output drive optic.py:599-602.
 #include "common.h"
 #include "groups.h"
+#include "panel.h"
 
 namespace {
 
@@ -60,8 +61,9 @@ __global__ __launch_bounds__(256) void k_optic_update(OpticArgs s, int n, int64_
 // One substep fused with its recurrent product (flyverse/optic.py:474-483 then 525-551): y = W_rr dr_cur for the
 // rows of group g (grouped order of launch_spmv_grouped), then each row's update as k_optic_update computes it, the
 // next dr into dr_next (a different buffer: other rows' products still read dr_cur).  One wavefront per (group, fly).
+template <class Matrix>
 struct SubstepArgs {
-    fh_groups::Grouped rr;
+    Matrix rr;
     const float* dr_cur;
     float *dr_next, *v, *adapt;
     const float *pr, *spk, *a, *b;
@@ -69,7 +71,27 @@ struct SubstepArgs {
     float gr, ga, aad;
 };
 
-__global__ __launch_bounds__(256) void k_optic_substep(SubstepArgs s) {
+// The update of row i of the fly whose state starts at `base`, given its recurrent product total.
+template <class Matrix>
+__device__ __forceinline__ void substep_update(const SubstepArgs<Matrix>& s, int64_t base, int i, float total) {
+    const int64_t t = base + i;
+    const float old = s.dr_cur[t];
+    float inp = s.gr * total;
+    inp = inp + s.pr[t];
+    float gad = s.ga * s.adapt[t];
+    inp = inp - gad;
+    inp = inp + s.spk[t];
+    float d = s.v[t] - inp;
+    d = d * s.a[i];
+    float v = inp + d;
+    s.v[t] = v;
+    float da = s.adapt[t] - old;
+    da = da * s.aad;
+    s.adapt[t] = old + da;
+    s.dr_next[t] = dr_value(v, s.b[i]);
+}
+
+__global__ __launch_bounds__(256) void k_optic_substep(SubstepArgs<fh_groups::Grouped> s) {
     __shared__ float sums[4][kWave];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -77,23 +99,17 @@ __global__ __launch_bounds__(256) void k_optic_substep(SubstepArgs s) {
     if (g >= s.rr.ngroups) return;
     const int64_t base = (int64_t)blockIdx.y * s.n_rate;
     const float* x = s.dr_cur + base;
-    fh_groups::group_product(s.rr, g, lane, sums[wv], [&](int i) { return x[i]; }, [&](int i, float total) {
-        const int64_t t = base + i;
-        const float old = s.dr_cur[t];
-        float inp = s.gr * total;
-        inp = inp + s.pr[t];
-        float gad = s.ga * s.adapt[t];
-        inp = inp - gad;
-        inp = inp + s.spk[t];
-        float d = s.v[t] - inp;
-        d = d * s.a[i];
-        float v = inp + d;
-        s.v[t] = v;
-        float da = s.adapt[t] - old;
-        da = da * s.aad;
-        s.adapt[t] = old + da;
-        s.dr_next[t] = dr_value(v, s.b[i]);
-    });
+    fh_groups::group_product(s.rr, g, lane, sums[wv], [&](int i) { return x[i]; },
+                             [&](int i, float total) { substep_update(s, base, i, total); });
+}
+
+// The same substep with the W_rr product by persistent panels (panel.h): one 1024-thread workgroup per (row block,
+// fly), x = dr_cur staged in LDS a panel at a time; each row's update runs when its total is folded.
+__global__ __launch_bounds__(1024) void k_optic_substep_panel(SubstepArgs<fh_panel::Panel> s) {
+    extern __shared__ float xs[];
+    const int64_t base = (int64_t)blockIdx.y * s.n_rate;
+    fh_panel::block_product(s.rr, blockIdx.x, s.dr_cur + base, xs,
+                            [&](int i, float total) { substep_update(s, base, i, total); });
 }
 
 // Photoreceptor stage over (column, family) cells (flyverse/optic.py:496-517), one thread per (fly b, cell k),
@@ -336,11 +352,32 @@ FH_EXPORT int launch_optic_substep(void** p, int n_rate, int ngroups, int batch,
                                    void* stream) {
     if (n_rate < 0 || ngroups < 0 || batch < 0 || batch > 65535) return FH_ERR_ARGS;
     if (!n_rate || !ngroups || !batch) return 0;
-    SubstepArgs s{fh_groups::Grouped{(const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3],
-                                     ngroups},
+    SubstepArgs<fh_groups::Grouped> s{fh_groups::Grouped{(const int*)p[0], (const int*)p[1], (const int4*)p[2],
+                                                         (const float4*)p[3], ngroups},
                   (const float*)p[4], (float*)p[5], (float*)p[6], (float*)p[7], (const float*)p[8],
                   (const float*)p[9], (const float*)p[10], (const float*)p[11], n_rate, gr, ga, aad};
     hipLaunchKernelGGL(k_optic_substep, dim3((ngroups + 3) / 4, batch), dim3(256), 0, (hipStream_t)stream, s);
+    FH_LAUNCH_CHECK();
+}
+
+// The substep with its W_rr product by panels: p = [W_rr blk_row (nblk+1; blocks of launch_spmv_panel), chunk_ptr,
+// pidx, pw, dr_cur, dr_next, v, adapt, pr, spk (B, n_rate each), a, b (n_rate)].  Same values as
+// launch_optic_substep.
+FH_EXPORT int launch_optic_substep_panel(void** p, int n_rate, int nblk, int batch, float gr, float ga, float aad,
+                                         void* stream) {
+    if (n_rate < 0 || nblk < 0 || batch < 0 || batch > 65535) return FH_ERR_ARGS;
+    if (!n_rate || !nblk || !batch) return 0;
+    const int pc = fh_panel::panel_columns(n_rate, 0);
+    const int lds_floats = max(pc, fh_panel::kChunks * fh_panel::kThreads);
+    hipError_t e = hipFuncSetAttribute((const void*)k_optic_substep_panel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_floats * 4);
+    if (e != hipSuccess) return (int)e;
+    SubstepArgs<fh_panel::Panel> s{fh_panel::Panel{(const int*)p[0], (const int*)p[1], (const int4*)p[2],
+                                                   (const float4*)p[3], n_rate, pc},
+                                   (const float*)p[4], (float*)p[5], (float*)p[6], (float*)p[7], (const float*)p[8],
+                                   (const float*)p[9], (const float*)p[10], (const float*)p[11], n_rate, gr, ga, aad};
+    hipLaunchKernelGGL(k_optic_substep_panel, dim3(nblk, batch), dim3(fh_panel::kThreads), (size_t)lds_floats * 4,
+                       (hipStream_t)stream, s);
     FH_LAUNCH_CHECK();
 }
 

@@ -1,0 +1,132 @@
+// Panel product of one fly's row block, shared by the B = 1 product kernel and the fused optic substep.
+//
+// Storage: the chunk-padded CSR of groups.h (chunk c = entries [16 c, 16 c + 16) of pidx / pw, pads idx -1, entries
+// of a row in ascending column order), cut on the host into blocks of consecutive rows of at most kPanelChunks
+// chunks per thread.  One 1024-thread workgroup works on one (block, fly) for the whole launch: thread t holds the
+// indices and weights of the block's chunks t, t + 1024, t + 2048 in registers.  x is staged in LDS one panel of
+// columns at a time; in each pass a thread adds, in entry order, the products of its chunks' entries whose column
+// lies in the panel.  The entries of a chunk ascend in column and the panels are visited in ascending order, so
+// each chunk is still one sequential fold of its products from +0, in entry order.  The chunk sums then go through
+// LDS and one thread per row folds them in chunk order: the same additions in the same order as the grouped
+// product (groups.h) and launch_csr.
+//
+// The point of the panels: a 4-byte gather through the vector caches moves a whole 128-byte line from L2 into the
+// L1, one line per entry; from LDS it moves 4 bytes.  Each workgroup reads x once per launch, so the HBM stream of
+// the CSR itself (8 bytes per entry) is what is left.
+#pragma once
+#include "common.h"
+
+namespace fh_panel {
+
+constexpr int kThreads = 1024;        // threads per workgroup
+constexpr int kChunks = 3;            // chunks per thread
+constexpr int kMaxPanel = 40000;      // floats of x per panel (160,000 bytes of LDS)
+
+// Floats of x per panel: `panel` if given (at most kMaxPanel), else the columns cut into as few panels as fit, of
+// equal size rounded up to a multiple of 64.
+inline int panel_columns(int cols, int panel) {
+    if (panel > 0) return min(panel, kMaxPanel);
+    const int n = max(1, (cols + kMaxPanel - 1) / kMaxPanel);
+    const int pc = ((cols + n - 1) / n + 63) / 64 * 64;
+    return max(64, min(pc, kMaxPanel));
+}
+
+struct Panel {
+    const int* blk_row;    // first row of each block, nblk + 1
+    const int* chunk_ptr;  // rows + 1
+    const int4* pidx;
+    const float4* pw;
+    int cols, panel;       // columns of x, floats per panel (also at least kChunks * kThreads, the chunk sums)
+};
+
+// y-row results of block `blk` for the fly whose x row is xb; xs is the dynamic LDS buffer (max(panel,
+// kChunks * kThreads) floats).  finish(row, total) runs once per row of the block, on some thread.
+template <class Finish>
+__device__ __forceinline__ void block_product(const Panel& m, int blk, const float* __restrict__ xb, float* xs,
+                                              Finish finish) {
+    const int t = threadIdx.x;
+    const int r0 = m.blk_row[blk], r1 = m.blk_row[blk + 1];
+    const int cb = m.chunk_ptr[r0], ce = m.chunk_ptr[r1];
+    int iv[kChunks][16];
+    float wv[kChunks][16];
+#pragma unroll
+    for (int u = 0; u < kChunks; ++u) {
+        const int c = cb + t + u * kThreads;
+        if (c < ce) {
+            const int4* ip = m.pidx + (int64_t)c * 4;
+            const float4* wp = m.pw + (int64_t)c * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int4 i4 = ip[q];
+                const float4 w4 = wp[q];
+                iv[u][4 * q] = i4.x; iv[u][4 * q + 1] = i4.y; iv[u][4 * q + 2] = i4.z; iv[u][4 * q + 3] = i4.w;
+                wv[u][4 * q] = w4.x; wv[u][4 * q + 1] = w4.y; wv[u][4 * q + 2] = w4.z; wv[u][4 * q + 3] = w4.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) { iv[u][j] = -1; wv[u][j] = 0.0f; }
+        }
+    }
+    float s[kChunks];
+#pragma unroll
+    for (int u = 0; u < kChunks; ++u) s[u] = 0.0f;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    for (int lo = 0; lo < m.cols; lo += m.panel) {
+        const int n = min(m.panel, m.cols - lo);
+        __syncthreads();                               // the previous panel is no longer read
+        // direct-to-LDS loads (global_load_lds_dword: lane i of a wave writes 4 bytes at the wave's LDS base + 4 i),
+        // 64 floats per wave and instruction, all of a panel in flight at once without registers
+        for (int base = wave * 64; base < n; base += kThreads) {
+            if (base + lane < n)
+                __builtin_amdgcn_global_load_lds((const void*)(xb + lo + base + lane),
+                                                 (__attribute__((address_space(3))) void*)(xs + base), 4, 0, 0);
+        }
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        // the reads of R entries are issued before their adds (an entry outside the panel reads xs[0] and is not
+        // added), so the LDS latency is paid once per R entries instead of once per entry
+        constexpr int R = 4;
+        const unsigned un = (unsigned)n;
+#pragma unroll
+        for (int u = 0; u < kChunks; ++u) {
+#pragma unroll
+            for (int j0 = 0; j0 < 16; j0 += R) {
+                float xv[R];
+                bool ok[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const unsigned off = (unsigned)(iv[u][j0 + j] - lo);   // in the panel iff off < n; pads never
+                    ok[j] = off < un;
+                    xv[j] = xs[ok[j] ? off : 0u];
+                }
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const float c = wv[u][j0 + j] * xv[j];
+                    const float a = s[u] + c;
+                    s[u] = ok[j] ? a : s[u];
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kChunks; ++u) xs[t + u * kThreads] = s[u];     // chunk cb + i's sum at xs[i]
+    __syncthreads();
+    for (int row = r0 + t; row < r1; row += kThreads) {
+        const int ca = m.chunk_ptr[row] - cb, cz = m.chunk_ptr[row + 1] - cb;
+        float total = 0.0f;
+        int c = ca;
+        for (; c + 8 <= cz; c += 8) {                  // eight reads in flight, then their adds in order
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = xs[c + q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) total = total + v[q];
+        }
+        for (; c < cz; ++c) total = total + xs[c];
+        finish(row, total);
+    }
+}
+
+}  // namespace fh_panel

@@ -6,6 +6,7 @@
 // order (torch sparse mm) is not pinned by anything it ships; agreement with it is to 1e-5 relative.
 #include "common.h"
 #include "groups.h"
+#include "panel.h"
 
 #include <mutex>
 
@@ -192,6 +193,20 @@ __global__ __launch_bounds__(1024) void k_spmv_lds(const int* __restrict__ tile_
     }
 }
 
+// Persistent panel product (panel.h): one 1024-thread workgroup per (row block, fly), the block's chunks in
+// registers, x staged in LDS one panel of columns at a time.  Same additions, same order as k_csr.
+__global__ __launch_bounds__(1024) void k_spmv_panel(fh_panel::Panel m, const float* __restrict__ x,
+                                                     float* __restrict__ y, int rows, float scale, int accumulate) {
+    extern __shared__ float xs[];
+    const int b = blockIdx.y;
+    fh_panel::block_product(m, blockIdx.x, x + (int64_t)b * m.cols, xs, [&](int row, float total) {
+        if (scale != 1.0f) total = total * scale;
+        float* yp = y + (int64_t)b * rows + row;
+        if (accumulate) total = *yp + total;
+        *yp = total;
+    });
+}
+
 }  // namespace
 
 // Same contract as flyverse/kernels/neural.cu:105 (launch_csr): p = [ptr int32, idx int32, values (float or half),
@@ -246,6 +261,26 @@ FH_EXPORT int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batc
     hipLaunchKernelGGL(k_spmv_lds, dim3((unsigned)ntiles, (unsigned)batch), dim3(1024), (size_t)rc * 4,
                        (hipStream_t)stream, (const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3],
                        (const float*)p[4], (float*)p[5], rows, cols, rc, scale, accumulate);
+    FH_LAUNCH_CHECK();
+}
+
+// Panel product, same order as launch_csr, over the chunk-padded storage of launch_spmv_grouped: p = [blk_row
+// (nblk+1): first row of each block (at most 3 * 1024 chunks), chunk_ptr (rows+1), pidx (16*nchunks), pw
+// (16*nchunks), x (B, cols), y (B, rows)]; panel = columns of x staged at a time (0: as many as fit, balanced).
+// y = scale * W x (+ y if accumulate).
+FH_EXPORT int launch_spmv_panel(void** p, int rows, int cols, int nblk, int batch, int panel, float scale,
+                                int accumulate, void* stream) {
+    if (rows < 0 || cols < 0 || batch < 0 || nblk < 0 || batch > 65535 || panel < 0) return FH_ERR_ARGS;
+    if ((int64_t)rows * batch == 0 || nblk == 0) return 0;
+    const int pc = fh_panel::panel_columns(cols, panel);
+    const int lds_floats = max(pc, fh_panel::kChunks * fh_panel::kThreads);
+    hipError_t e = hipFuncSetAttribute((const void*)k_spmv_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       lds_floats * 4);
+    if (e != hipSuccess) return (int)e;
+    const fh_panel::Panel m{(const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3], cols, pc};
+    hipLaunchKernelGGL(k_spmv_panel, dim3((unsigned)nblk, (unsigned)batch), dim3(fh_panel::kThreads),
+                       (size_t)lds_floats * 4, (hipStream_t)stream, m, (const float*)p[4], (float*)p[5], rows, scale,
+                       accumulate);
     FH_LAUNCH_CHECK();
 }
 

@@ -110,6 +110,15 @@ int launch_spmv_grouped(void** p, int rows, int cols, int ngroups, int batch, fl
 int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batch, int range_cols, float scale, int accumulate,
                     void* stream);
 
+/* The launch_csr product by persistent panels: one 1024-thread workgroup per (row block, fly) holds the block's
+ * chunks (at most 3 per thread) in registers for the whole launch, and x is staged in LDS one panel of columns at a
+ * time, each workgroup reading x once; same additions in the same order as launch_csr.  p = [blk_row (nblk+1): first
+ * row of each block of at most 3 * 1024 chunks, chunk_ptr (rows+1), pidx (16*nchunks), pw (16*nchunks), x (batch,
+ * cols), y (batch, rows)]; panel = columns staged at a time (0: as few equal panels as the LDS allows).
+ * y = scale * W x (+ y if accumulate).  Replaces the _mv products of flyverse/optic.py:161-165. */
+int launch_spmv_panel(void** p, int rows, int cols, int nblk, int batch, int panel, float scale, int accumulate,
+                      void* stream);
+
 /* Photoreceptor stage per (column, family) cell (flyverse/optic.py:496-517): the averaging matrix of
  * optic.py:320-325 (cell_w = 1/count) over the photoreceptor intensities, low-pass, adaptation and contrast (0 for an
  * empty cell), each photoreceptor's activity = its cell's contrast.  p = [rad (batch, n_col, 4), sens (n_pr, 4),
@@ -128,6 +137,13 @@ int launch_drive_out(void** p, int n, int n_spk, int batch, float gain_out, floa
  * into dr_next (another buffer).  p = [W_rr group_row, chunk_ptr, pidx, pw, dr_cur, dr_next, v, adapt, pr, spk
  * (batch, n_rate each), a, b (n_rate)]; ngroups = W_rr groups.  Replaces a launch_csr + launch_optic pair. */
 int launch_optic_substep(void** p, int n_rate, int ngroups, int batch, float gr, float ga, float aad, void* stream);
+
+/* launch_optic_substep with the W_rr product by persistent panels (launch_spmv_panel's blocks and order, which are
+ * launch_csr's additions): p = [W_rr blk_row (nblk+1), chunk_ptr, pidx, pw, dr_cur, dr_next, v, adapt, pr, spk
+ * (batch, n_rate each), a, b (n_rate)].  Same values as launch_optic_substep.  Replaces a launch_csr + launch_optic
+ * pair (flyverse/optic.py:474-483 then 525-551). */
+int launch_optic_substep_panel(void** p, int n_rate, int nblk, int batch, float gr, float ga, float aad,
+                               void* stream);
 
 /* The frame's optic inputs (flyverse/optic.py:496-517 photoreceptor stage, :572 feedback, :579-580 the held
  * inputs) in two launches: the photoreceptor cells (as launch_photoreceptor), s = clamp(rate[:, spk_idx] / 100, 0, 3)

@@ -50,11 +50,17 @@ def _check_state(ol, os_, spec, what):
     assert_bits(N(ol._fresh), os_.fresh, f"fresh {what}")
 
 
-def test_fused_frames_match_the_oracle_frames():
+@pytest.mark.parametrize("substep", ["panel", "grouped"])
+def test_fused_frames_match_the_oracle_frames(substep):
     """Twelve frames of B=3 flies: 10 ms (10 substeps), 0.4 ms (none; the remainder carries), 2.5 ms (2 or 3 with
-    the remainder), with a fly reset after frame 4; drive, optic state, photoreceptor state and delta_rate bit-exact."""
+    the remainder), with a fly reset after frame 4; drive, optic state, photoreceptor state and delta_rate bit-exact.
+    Both substep kernels: the W_rr product by persistent panels (the default) and by row groups."""
+    from flyhip import hip
     B = 3
     conn, retina, ol, spec, os_ = _setup(B)
+    if substep != "panel":
+        ol.W_rr = hip.CSR(ol.matrices["M_rr"], DEV, kernel=substep)
+    assert ol.W_rr.kernel == substep
     rng = np.random.default_rng(11)
     lengths = [10.0, 0.4, 2.5, 10.0, 0.4, 0.4, 2.5, 10.0, 2.5, 0.4, 10.0, 10.0]
     substeps = 0

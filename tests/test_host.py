@@ -361,3 +361,18 @@ def test_tuner_cache_key_ring_layout_and_refused_shapes(monkeypatch):
     small = SimpleNamespace(**vars(fake("slot")) | {"n_delay": 3})
     with pytest.raises(ValueError, match="no launch shape to try"):
         autotune.tune_batch(small, steps=(4, 5, 6), use_cache=False)
+
+
+def test_panel_blocks_cover_the_rows_within_the_register_budget():
+    """The row blocks of the panel product: consecutive, covering every row once, at most 3 * 1024 chunks each, at
+    least one block per compute unit when the rows allow, and no blocks when one row alone is too long."""
+    rng = np.random.default_rng(3)
+    cap = hip.PANEL_THREADS * hip.PANEL_CHUNKS
+    for counts, n_cu in ((rng.integers(0, 40, 90000), 256), (rng.integers(1, 300, 5000), 256),
+                         (np.array([3000, 3072, 1, 0, 2000]), 2), (np.zeros(7, np.int64), 256), (np.ones(3), 256)):
+        starts = hip.panel_blocks(counts, n_cu)
+        assert starts[0] == 0 and starts[-1] == len(counts) and np.all(np.diff(starts) > 0)
+        sums = np.add.reduceat(counts, starts[:-1]) if len(counts) else np.zeros(0)
+        assert sums.max(initial=0) <= cap
+        assert len(starts) - 1 >= min(n_cu, len(counts))
+    assert hip.panel_blocks(np.array([5, cap + 1, 5]), 256) is None
