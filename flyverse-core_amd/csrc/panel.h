@@ -75,17 +75,29 @@ __device__ __forceinline__ void block_product(const Panel& m, int blk, const flo
     for (int lo = 0; lo < m.cols; lo += m.panel) {
         const int n = min(m.panel, m.cols - lo);
         __syncthreads();                               // the previous panel is no longer read
-        // direct-to-LDS loads (global_load_lds_dword: lane i of a wave writes 4 bytes at the wave's LDS base + 4 i),
-        // 64 floats per wave and instruction, all of a panel in flight at once without registers
-        for (int base = wave * 64; base < n; base += kThreads) {
-            if (base + lane < n)
-                __builtin_amdgcn_global_load_lds((const void*)(xb + lo + base + lane),
-                                                 (__attribute__((address_space(3))) void*)(xs + base), 4, 0, 0);
+        // direct-to-LDS loads (global_load_lds: lane i of a wave writes its 16 or 4 bytes at the wave's LDS base +
+        // 16 i or 4 i), all of a panel in flight at once without registers; 16-byte loads when the panel's start in
+        // x is 16-byte aligned, the last n mod 4 floats by plain loads and stores
+        if ((reinterpret_cast<uintptr_t>(xb + lo) & 15) == 0) {
+            const int n4 = n & ~3;
+            for (int base = wave * 256; base < n4; base += kThreads * 4) {
+                if (base + 4 * lane < n4)
+                    __builtin_amdgcn_global_load_lds((const void*)(xb + lo + base + 4 * lane),
+                                                     (__attribute__((address_space(3))) void*)(xs + base), 16, 0, 0);
+            }
+            if (t < n - n4) xs[n4 + t] = xb[lo + n4 + t];
+        } else {
+            for (int base = wave * 64; base < n; base += kThreads) {
+                if (base + lane < n)
+                    __builtin_amdgcn_global_load_lds((const void*)(xb + lo + base + lane),
+                                                     (__attribute__((address_space(3))) void*)(xs + base), 4, 0, 0);
+            }
         }
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         // the reads of R entries are issued before their adds (an entry outside the panel reads xs[0] and is not
-        // added), so the LDS latency is paid once per R entries instead of once per entry
+        // added), so the LDS latency is paid once per R entries instead of once per entry; a group of R entries
+        // that no lane of the wave has in this panel is skipped
         constexpr int R = 4;
         const unsigned un = (unsigned)n;
 #pragma unroll
@@ -94,12 +106,17 @@ __device__ __forceinline__ void block_product(const Panel& m, int blk, const flo
             for (int j0 = 0; j0 < 16; j0 += R) {
                 float xv[R];
                 bool ok[R];
+                unsigned off[R];
+                bool any = false;
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
-                    const unsigned off = (unsigned)(iv[u][j0 + j] - lo);   // in the panel iff off < n; pads never
-                    ok[j] = off < un;
-                    xv[j] = xs[ok[j] ? off : 0u];
+                    off[j] = (unsigned)(iv[u][j0 + j] - lo);   // in the panel iff off < n; pads (-1) never
+                    ok[j] = off[j] < un;
+                    any = any || ok[j];
                 }
+                if (!__builtin_amdgcn_ballot_w64(any)) continue;
+#pragma unroll
+                for (int j = 0; j < R; ++j) xv[j] = xs[ok[j] ? off[j] : 0u];
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
                     const float c = wv[u][j0 + j] * xv[j];
@@ -117,12 +134,12 @@ __device__ __forceinline__ void block_product(const Panel& m, int blk, const flo
         const int ca = m.chunk_ptr[row] - cb, cz = m.chunk_ptr[row + 1] - cb;
         float total = 0.0f;
         int c = ca;
-        for (; c + 8 <= cz; c += 8) {                  // eight reads in flight, then their adds in order
-            float v[8];
+        for (; c + 16 <= cz; c += 16) {                // sixteen reads in flight, then their adds in order
+            float v[16];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = xs[c + q];
+            for (int q = 0; q < 16; ++q) v[q] = xs[c + q];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) total = total + v[q];
+            for (int q = 0; q < 16; ++q) total = total + v[q];
         }
         for (; c < cz; ++c) total = total + xs[c];
         finish(row, total);
