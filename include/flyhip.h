@@ -311,6 +311,11 @@ int launch_segment_sum(void** p, int nseg, void* stream);
 /* out[0] = scale * fixed-order sum of n partial losses.  p = [parts, out]. */
 int launch_loss_total(void** p, int64_t n, float scale, void* stream);
 
+/* A bandwidth probe, not part of the brain step and with no counterpart in the reference: y = x for 4 * n4 floats
+ * (16-byte aligned, not overlapping) by a streaming copy with non-temporal loads and stores, `grid` workgroups of
+ * 256 threads.  bench.py times it to state the copy bandwidth this GPU attains beside the vendor peak. */
+int launch_copy_stream(const void* x, void* y, int64_t n4, int grid, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
