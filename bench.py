@@ -344,6 +344,60 @@ def attainable_copy_gbs(dev, reps=10):
     return best
 
 
+def c0_run(cpu: bool, steps: int = 1000):
+    """The plain-LIF workload in full (16,384 neurons, lognormal out-degree, 200 Poisson inputs at 150 Hz, dt 0.1 ms,
+    1,000 steps; synth.plain_lif_graph, the configuration of the committed C0 golden): the plain HIP path (Brain,
+    windows of up to 18 steps) timed with HIP events after 100 untimed steps, and with `cpu` the CPU path from rest
+    on all host cores, the C oracle (OpenMP) and the reference's step in torch float32 ops, each for all `steps`."""
+    import torch
+    from flyhip import synth
+    from flyhip.brain import Brain, LIFParams
+    c, inputs, hz = synth.plain_lif_graph(0, n=16_384, n_inputs=200, hz=150.0)
+    lp = LIFParams.shiu(dt=0.1)
+    brain = Brain(c, lp, device="cuda", batch=1, seed=20240607)
+    brain.set_poisson(inputs, hz)
+    brain.step(100)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    brain.step(steps)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    out = {"c0_neurons": int(c.n), "c0_edges": int(c.W.nnz), "c0_steps": steps, "c0_gpu_ms": round(ms, 3),
+           "c0_gpu_sim_s_per_wall_s": round(steps * lp.dt / ms, 4)}
+    del brain
+    if cpu:
+        import scipy.sparse as sp
+        sys.path.insert(0, str(ROOT))
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle
+        from flyhip.brain import shaped_weights
+        from helpers import oracle_fly
+        from oracle.torch_ref import RefBrain
+        cores = len(os.sched_getaffinity(0))
+        oracle.set_threads(cores)
+        om = oracle_fly(c, None, lp, None, 1, [20240607], None, with_optic=False)
+        om.st.poisson_p[0, inputs] = (hz * np.float32(lp.dt / 1000.0)).astype(np.float32)
+        om.poisson_cells = np.asarray(inputs, np.int32)
+        t0 = time.perf_counter()
+        for _ in om.lif_steps(steps):
+            pass
+        wall_o = time.perf_counter() - t0
+        torch.set_num_threads(cores)
+        rb = RefBrain(sp.csr_matrix(shaped_weights(c, lp)), 1, lp, np.ones(c.n, np.float32))
+        rb.poisson_p[0, torch.from_numpy(np.asarray(inputs, np.int64))] = torch.from_numpy(hz) * (lp.dt / 1000.0)
+        gen = torch.Generator().manual_seed(0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rb.step(torch.rand(1, c.n, generator=gen))
+        wall_t = time.perf_counter() - t0
+        out.update({"c0_cpu_cores": cores, "c0_cpu_oracle_openmp_ms": round(wall_o * 1e3, 1),
+                    "c0_cpu_torch_reference_ops_ms": round(wall_t * 1e3, 1),
+                    "c0_gpu_vs_cpu": round(min(wall_o, wall_t) * 1e3 / ms, 1)})
+    return out
+
+
 def cpu_baseline(conn, retina, lp, op, rollouts, oracle_frames, torch_frames):
     """The CPU path on every core of this host, B = len(rollouts) flies of the workload from rest:
       * the C oracle (oracle/, OpenMP over elements, rows and flies) for `oracle_frames` frames;
@@ -411,7 +465,7 @@ def main():
                     help="pin the B=64 launch shape (fold mode, steps per launch, workgroups per CU); skips tuning")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-optic-training", action="store_true")
-    ap.add_argument("--only", choices=("b64", "spmv", "b1", "optic-training", "packed"), default=None,
+    ap.add_argument("--only", choices=("b64", "spmv", "b1", "optic-training", "packed", "c0"), default=None,
                     help="run one part alone (plus --copy-calibration) and print its figures: the short runs the "
                          "kernel traces and PMC counter passes profile")
     ap.add_argument("--only-b64", action="store_true", help="the same as --only b64")
@@ -442,7 +496,8 @@ def main():
     if args.only not in (None, "b64"):
         part = {"spmv": lambda: spmv_roofline(conn, retina, op), "b1": lambda: b1_run(conn, retina, lp, op, rollouts[0],
                                                                                         args.b1_frames),
-                "optic-training": optic_training, "packed": lambda: packed_run(args.packed_steps, log)}[args.only]()
+                "optic-training": optic_training, "packed": lambda: packed_run(args.packed_steps, log),
+                "c0": lambda: c0_run(not args.no_cpu)}[args.only]()
         if args.copy_calibration:
             copy_calibration(dev)
         print(json.dumps(part), flush=True)
@@ -520,6 +575,12 @@ def main():
             f"({ol['ol_us_per_simulated_step']} us per simulated step)")
         torch.cuda.empty_cache()
 
+    c0 = c0_run(not args.no_cpu)
+    log(f"C0 plain LIF (16,384 neurons, 1,000 steps): GPU {c0['c0_gpu_ms']} ms"
+        + (f", CPU oracle {c0['c0_cpu_oracle_openmp_ms']} ms, torch reference ops "
+           f"{c0['c0_cpu_torch_reference_ops_ms']} ms on {c0['c0_cpu_cores']} cores" if "c0_cpu_cores" in c0 else ""))
+    torch.cuda.empty_cache()
+
     packed = None
     if not args.no_packed:
         log("packed uneven subgraphs ...")
@@ -557,7 +618,7 @@ def main():
                    "attainable_copy_gbs": round(copy_gbs, 1), "b1_sim_s_per_wall_s": b1, "ratio_b64_vs_b1": round(value / b1, 2),
                    "b1_ms_per_frame": b1_ms_frame, "spike_activity_per_step": activity,
                    "b64_launch_shape": tuned, "optic_substep_b64_us": round(sub_us, 2),
-                   "optic_substep_b64_frac": round(sub_frac, 4), **spmv, **(ol or {}), **(packed or {})},
+                   "optic_substep_b64_frac": round(sub_frac, 4), **spmv, **(ol or {}), **(packed or {}), **c0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "attainable": round(copy_gbs, 1), "frac_attainable": round(achieved / copy_gbs, 4),
