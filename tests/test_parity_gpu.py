@@ -243,6 +243,25 @@ def test_csr_products_bit_exact_and_close_to_float64(B, kernel, range_cols):
     assert_bits(N_(h.matvec(T(x))), oracle.spmv(M.indptr, M.indices, wh, x, half=True), "half")
 
 
+def test_copy_probe_copies_and_refuses_misaligned_buffers():
+    """The bandwidth probe bench.py times (launch_copy_stream): an exact copy of 4 * n4 floats at any grid, the
+    floats past them untouched, and an argument error (nothing launched) for a buffer that is not 16-byte aligned."""
+    lib = hip.lib(DEV)
+    stream = torch.cuda.current_stream().cuda_stream
+    x = torch.randn(4 * 1000 + 8, device=DEV)
+    for grid in (1, 7, 256):
+        y = torch.full_like(x, -1.0)
+        assert lib.launch_copy_stream(x.data_ptr(), y.data_ptr(), 1000, grid, stream) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(y[:4000], x[:4000]) and bool((y[4000:] == -1.0).all())
+    y = torch.zeros_like(x)
+    assert lib.launch_copy_stream(x[1:].data_ptr(), y.data_ptr(), 10, 4, stream) < 0
+    assert lib.launch_copy_stream(x.data_ptr(), y.data_ptr(), 10, 0, stream) < 0
+    assert lib.launch_copy_stream(x.data_ptr(), y.data_ptr(), 0, 4, stream) == 0
+    torch.cuda.synchronize()
+    assert not bool(y.any())
+
+
 def test_csr_empty_and_validation():
     csr = hip.CSR(sp.csr_matrix((3, 4), dtype=np.float32), DEV)
     x = torch.zeros(2, 4, device=DEV)
