@@ -368,7 +368,7 @@ FH_EXPORT int launch_optic_substep_panel(void** p, int n_rate, int nblk, int bat
     if (n_rate < 0 || nblk < 0 || batch < 0 || batch > 65535) return FH_ERR_ARGS;
     if (!n_rate || !nblk || !batch) return 0;
     const int pc = fh_panel::panel_columns(n_rate, 0);
-    const int lds_floats = max(pc, fh_panel::kChunks * fh_panel::kThreads);
+    const int lds_floats = fh_panel::lds_floats(pc);
     hipError_t e = hipFuncSetAttribute((const void*)k_optic_substep_panel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds_floats * 4);
     if (e != hipSuccess) return (int)e;

@@ -273,7 +273,7 @@ FH_EXPORT int launch_spmv_panel(void** p, int rows, int cols, int nblk, int batc
     if (rows < 0 || cols < 0 || batch < 0 || nblk < 0 || batch > 65535 || panel < 0) return FH_ERR_ARGS;
     if ((int64_t)rows * batch == 0 || nblk == 0) return 0;
     const int pc = fh_panel::panel_columns(cols, panel);
-    const int lds_floats = max(pc, fh_panel::kChunks * fh_panel::kThreads);
+    const int lds_floats = fh_panel::lds_floats(pc);
     hipError_t e = hipFuncSetAttribute((const void*)k_spmv_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        lds_floats * 4);
     if (e != hipSuccess) return (int)e;

@@ -210,7 +210,8 @@ def test_brain_synaptic_input_accumulates_in_float32(dtype):
 @pytest.mark.parametrize("B,kernel,range_cols", [(1, "grouped", 0), (3, "grouped", 0), (8, "grouped", 0),
                                                   (1, "chunked", 0), (3, "chunked", 0), (1, "lds", 0),
                                                   (3, "lds", 1024), (2, "lds", 1500), (1, "panel", 0),
-                                                  (3, "panel", 512), (2, "panel", 1500), (3, "panel", 999)])
+                                                  (3, "panel", 512), (2, "panel", 1500), (3, "panel", 999),
+                                                  (2, "panel", 333), (1, "panel", 64)])
 def test_csr_products_bit_exact_and_close_to_float64(B, kernel, range_cols):
     rng = np.random.default_rng(5 + B)
     rows, cols = 2000, 1700
