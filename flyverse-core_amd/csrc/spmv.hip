@@ -14,32 +14,121 @@ namespace {
 
 constexpr int kChunk = 16;
 
-// One wavefront per (fly, row) over the (B, cols) fly-major x: lane k folds chunk k of each round of 64 chunks,
-// then the wave folds the chunk sums in order.  With half weights x is rounded to half first (neural.cu:99).
+// One wavefront per (fly, window of kWinRows consecutive rows) over the (B, cols) fly-major x.  With half weights x
+// is rounded to half first (neural.cu:99).
+// Each 16-lane DPP row folds one chunk: fifteen rounds of a one-lane shift within the DPP row (row_shr:1, lane 0 of
+// the row taking +0) and an add, so lane j holds (((0 + c_0) + c_1) + ... + c_j) once round j is done; lanes past a
+// row's end hold -0.0, which leaves every sum unchanged.  When no row of the window has more than kLaneRow entries,
+// the window's chunks run as one stream, four a round, and each chunk sum is folded into its row's total in chunk
+// order.  Otherwise each row of the window runs alone: its entries on the lanes, 64 a round, up to kLaneRow entries,
+// one chunk per lane (folded by the lane) beyond.
+constexpr int kLaneRow = 4 * kWave;
+constexpr int kWinRows = 4;
+constexpr int kRowShr1 = 0x111;
+
+// the chunk fold of one 64-entry round: the sum of chunk q of the round ends in lane 16 q + 15
+__device__ __forceinline__ float fold_chunks16(float c) {
+    float s = 0.0f + c;
+#pragma unroll
+    for (int i = 1; i < kChunk; ++i) {
+        const float t = __int_as_float(
+            __builtin_amdgcn_update_dpp(__float_as_int(0.0f), __float_as_int(s), kRowShr1, 0xf, 0xf, false));
+        s = t + c;
+    }
+    return s;
+}
+
+template <typename W>
+__device__ __forceinline__ float csr_product(const W* __restrict__ w, float xv, int e) {
+    if (sizeof(W) == 2) xv = __half2float(__float2half_rn(xv));
+    return (float)w[e] * xv;
+}
+
+// The total of one row [start, end) (uniform), every lane returning it.
+template <typename W>
+__device__ __forceinline__ float csr_row(const int* __restrict__ idx, const W* __restrict__ w,
+                                         const float* __restrict__ xb, int start, int end, int lane) {
+    float total = 0.0f;
+    if (end - start <= kLaneRow) {
+        for (int base = start; base < end; base += kWave) {
+            const int e = base + lane;
+            const float c = e < end ? csr_product(w, xb[idx[e]], e) : -0.0f;
+            const float s = fold_chunks16(c);
+            const int nch = min(kWave / kChunk, (end - base + kChunk - 1) / kChunk);
+            for (int q = 0; q < nch; ++q) total = total + fh_readlane(s, q * kChunk + kChunk - 1);
+        }
+    } else {
+        for (int cb = start; cb < end; cb += kWave * kChunk) {
+            int cs = cb + lane * kChunk, ce = min(cs + kChunk, end);
+            float s = 0.0f;
+            for (int e = cs; e < ce; ++e) s = s + csr_product(w, xb[idx[e]], e);
+            int nch = min(kWave, (end - cb + kChunk - 1) / kChunk);
+            for (int j = 0; j < nch; ++j) total = total + fh_readlane(s, j);
+        }
+    }
+    return total;
+}
+
 template <typename W>
 __global__ __launch_bounds__(256) void k_csr(const int* __restrict__ ptr, const int* __restrict__ idx,
                                              const W* __restrict__ w, const float* __restrict__ x,
                                              float* __restrict__ y, int rows, int cols, int batch) {
-    int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (gw >= (int64_t)rows * batch) return;
-    int lane = threadIdx.x & 63;
-    int b = (int)(gw / rows), row = (int)(gw % rows);
+    const int nwin = (rows + kWinRows - 1) / kWinRows;
+    const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (gw >= (int64_t)nwin * batch) return;
+    const int lane = threadIdx.x & 63;
+    const int b = (int)(gw / nwin), r0 = (int)(gw % nwin) * kWinRows;
     const float* xb = x + (int64_t)b * cols;
-    int start = ptr[row], end = ptr[row + 1];
-    float total = 0.0f;
-    for (int cb = start; cb < end; cb += kWave * kChunk) {
-        int cs = cb + lane * kChunk, ce = min(cs + kChunk, end);
-        float s = 0.0f;
-        for (int e = cs; e < ce; ++e) {
-            float xv = xb[idx[e]];
-            if (sizeof(W) == 2) xv = __half2float(__float2half_rn(xv));
-            float c = (float)w[e] * xv;
-            s = s + c;
+    float* yb = y + (int64_t)b * rows;
+    const int nr = min(kWinRows, rows - r0);
+    // the window's rows [st_r, en_r), absent rows empty
+    const int p0 = ptr[r0];
+    const int p1 = nr > 0 ? ptr[r0 + 1] : p0;
+    const int p2 = nr > 1 ? ptr[r0 + 2] : p1;
+    const int p3 = nr > 2 ? ptr[r0 + 3] : p2;
+    const int p4 = nr > 3 ? ptr[r0 + 4] : p3;
+    const int longest = max(max(p1 - p0, p2 - p1), max(p3 - p2, p4 - p3));
+    if (longest > kLaneRow) {
+        const int pr[kWinRows + 1] = {p0, p1, p2, p3, p4};
+#pragma unroll
+        for (int r = 0; r < kWinRows; ++r) {
+            if (r < nr) {
+                const float total = csr_row(idx, w, xb, pr[r], pr[r + 1], lane);
+                if (lane == 0) yb[r0 + r] = total;
+            }
         }
-        int nch = min(kWave, (end - cb + kChunk - 1) / kChunk);
-        for (int j = 0; j < nch; ++j) total = total + fh_readlane(s, j);
+        return;
     }
-    if (lane == 0) y[gw] = total;
+    // chunk stream: the rows' chunk counts and their running offsets
+    const int k1 = (p1 - p0 + kChunk - 1) / kChunk;
+    const int k2 = k1 + (p2 - p1 + kChunk - 1) / kChunk, k3 = k2 + (p3 - p2 + kChunk - 1) / kChunk;
+    const int T = k3 + (p4 - p3 + kChunk - 1) / kChunk;
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+    for (int g0 = 0; g0 < T; g0 += kWave / kChunk) {
+        const int g = g0 + (lane >> 4);
+        const int r = (g >= k1) + (g >= k2) + (g >= k3);
+        const int rs = r == 0 ? p0 : r == 1 ? p1 : r == 2 ? p2 : p3;
+        const int re = r == 0 ? p1 : r == 1 ? p2 : r == 2 ? p3 : p4;
+        const int kr = r == 0 ? 0 : r == 1 ? k1 : r == 2 ? k2 : k3;
+        const int e = rs + (g - kr) * kChunk + (lane & (kChunk - 1));
+        const float c = (g < T && e < re) ? csr_product(w, xb[idx[e]], e) : -0.0f;
+        const float s = fold_chunks16(c);
+        const int nq = min(kWave / kChunk, T - g0);
+        for (int q = 0; q < nq; ++q) {
+            const int gq = g0 + q;
+            const float v = fh_readlane(s, q * kChunk + kChunk - 1);
+            if (gq < k1) t0 = t0 + v;
+            else if (gq < k2) t1 = t1 + v;
+            else if (gq < k3) t2 = t2 + v;
+            else t3 = t3 + v;
+        }
+    }
+    if (lane == 0) {
+        yb[r0] = t0;
+        if (nr > 1) yb[r0 + 1] = t1;
+        if (nr > 2) yb[r0 + 2] = t2;
+        if (nr > 3) yb[r0 + 3] = t3;
+    }
 }
 
 // Binned variant for the (B, cols) layout: one lane per chunk over a precomputed chunk table, then one lane per row
@@ -213,9 +302,9 @@ __global__ __launch_bounds__(1024) void k_spmv_panel(fh_panel::Panel m, const fl
 // x float (B, cols), y float (B, rows)].
 FH_EXPORT int launch_csr(void** p, int rows, int cols, int batch, int half, void* stream) {
     if (rows < 0 || cols < 0 || batch < 0) return FH_ERR_ARGS;
-    int64_t waves = (int64_t)rows * batch;
+    const int64_t waves = (int64_t)((rows + kWinRows - 1) / kWinRows) * batch;
     if (waves == 0) return 0;
-    unsigned blocks = (unsigned)((waves + 3) / 4);
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
     if (half)
         hipLaunchKernelGGL(k_csr<__half>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const int*)p[0],
                            (const int*)p[1], (const __half*)p[2], (const float*)p[3], (float*)p[4], rows, cols, batch);

@@ -263,6 +263,31 @@ def test_copy_probe_copies_and_refuses_misaligned_buffers():
     assert not bool(y.any())
 
 
+@pytest.mark.parametrize("B,half", [(1, False), (3, False), (2, True)])
+def test_reference_csr_product_over_row_windows(B, half):
+    """launch_csr (the neural.cu:105 contract) on rows of every length class of its windows of four rows: empty,
+    shorter than a chunk, one chunk exactly, just over, the 256-entry bound of a shared chunk stream and just over it
+    (the window's rows then run one at a time), in every position of a window, and a last partial window; bit-exact
+    against the oracle's chunked order (x rounded to half first for half weights)."""
+    rng = np.random.default_rng(31 + B)
+    cols = 700
+    lengths = [0, 1, 3, 16, 17, 64, 65, 255, 256, 257, 300, 2, 15, 0, 33, 48, 49, 256, 1, 5, 9]
+    lengths = lengths + lengths[5:] + lengths[:4]                  # 41 rows: the last window holds one
+    rows = []
+    for L in lengths:
+        c = np.sort(rng.choice(cols, L, replace=False))
+        rows.append(sp.csr_matrix((rng.standard_normal(L).astype(np.float32), c, [0, L]), shape=(1, cols)))
+    M = sp.vstack(rows).tocsr()
+    if half:
+        M.data = M.data.astype(np.float16).astype(np.float32)
+    x = rng.standard_normal((B, cols)).astype(np.float32)
+    x[:, ::5] = 0
+    x[:, 1::11] = -0.0
+    csr = hip.CSR(M, DEV, torch.float16 if half else torch.float32)
+    ref = oracle.spmv(M.indptr, M.indices, M.data, x, half=half)
+    assert_bits(N_(csr.matvec_reference_shape(T(x))), ref, "launch_csr")
+
+
 def test_csr_empty_and_validation():
     csr = hip.CSR(sp.csr_matrix((3, 4), dtype=np.float32), DEV)
     x = torch.zeros(2, 4, device=DEV)
