@@ -18,6 +18,7 @@ Shiu LIF at dt 0.1 ms coupled to the 89,390-unit optic lobe every 10 ms frame (1
     from flyhip.autotune.
   * cpu_baseline: the CPU path on all the host's cores, 4 flies of the same workload for a bounded sample of frames:
     the C oracle with OpenMP, and the reference's step in torch CPU ops (oracle.torch_ref); the faster is the value.
+  * --only abi (not part of the default run): the reference's six-entry C ABI at full scale, B = 1 and 8.
 """
 from __future__ import annotations
 
@@ -344,6 +345,77 @@ def attainable_copy_gbs(dev, reps=10):
     return best
 
 
+def abi_run(conn, retina, lp, op, batches=(1, 8), reps=20):
+    """The reference's six-entry C ABI (flyverse/kernels/neural.cu:54, 80, 84, 105, 124, 141, called through flyhip.hip,
+    the flyverse/cuda.py surface) on the full-scale graph (167,106 cells; the 89,390-unit optic lobe), at each batch
+    size: every entry point timed with HIP events around one replay of a graph of `reps` calls (3 for launch_events),
+    so the host's per-call work is not in the figure.
+    launch_lif runs with Poisson forcing on 300 cells and an injected uniform per (fly, cell); launch_events scatters
+    the spikes of 1.5 % of the cells through the Brain's CSC buffers and compact presynaptic list; launch_csr is the
+    W_rr product; launch_means averages 100 groups of 50 cells.  LIF algorithmic bytes per call: per (fly, cell) ten
+    4-byte fields read (v, g, drive, adapt, refrac, poisson_p, rnd, res, rate, spike_counts) and six written (v,
+    refrac, spikes, rate, spike_counts, the spike slot), per cell active and std_u (8 bytes)."""
+    import dataclasses
+    import torch
+    from flyhip import hip
+    from flyhip.brain import Brain
+    from flyhip.optic import OpticLobe, optic_matrices
+    dev = "cuda"
+    W_rr = hip.CSR(optic_matrices(conn, retina, op)["M_rr"], dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    out = {}
+
+    def timed(fn, r):
+        # r calls captured in one graph and replayed, so the figure is the GPU's time per call, not the host's
+        fn()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(r):
+                fn()
+        graph.replay()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(torch.cuda.current_stream())
+        graph.replay()
+        e1.record(torch.cuda.current_stream())
+        e1.synchronize()
+        del graph
+        return e0.elapsed_time(e1) * 1000 / r
+
+    for B in batches:
+        brain = Brain(conn, dataclasses.replace(lp, event_driven=True), device=dev, batch=B)
+        n = brain.n
+        brain.set_poisson(torch.randperm(n, generator=torch.Generator().manual_seed(3))[:300].numpy(),
+                          np.full(300, 150.0, np.float32))
+        rnd = torch.rand(B, n, device=dev, generator=gen)
+        x = (torch.rand(B, n, device=dev, generator=gen) < 0.015).float()
+        optic = OpticLobe(conn, retina, op, device=dev, batch=B)
+        m = optic.n_rate
+        y, pr, sk = (torch.rand(B, m, device=dev, generator=gen) for _ in range(3))
+        xr = torch.rand(B, m, device=dev, generator=gen)
+        gptr = torch.arange(0, 101 * 50, 50, dtype=torch.int32, device=dev)
+        gidx = torch.randperm(n, generator=torch.Generator().manual_seed(4))[:5000].to(torch.int32).to(dev)
+        calls = {
+            "lif": (lambda: hip.lif_update(brain, rnd), reps),
+            "events": (lambda: hip.event_scatter(brain._out_ptr, brain._out_post, brain._out_w, x, brain.g,
+                                                 brain._out_pre), 3),
+            "csr_wrr": (lambda: W_rr.matvec_reference_shape(xr), reps),
+            "optic_dr": (lambda: hip.optic_dr(optic.v, optic.b_vec, optic._cuda_dr), reps),
+            "optic": (lambda: hip.optic_update(optic, y, pr, sk), reps),
+            "means": (lambda: hip.group_means(gptr, gidx, brain.rate), reps),
+        }
+        for name, (fn, r) in calls.items():
+            out[f"abi_{name}_b{B}_us"] = round(timed(fn, r), 2)
+        lif_bytes = 4 * B * n * 16 + 8 * n
+        out[f"abi_lif_b{B}_frac"] = round(lif_bytes / (out[f"abi_lif_b{B}_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+        del brain, optic
+        torch.cuda.empty_cache()
+    out["abi_cells"], out["abi_optic_units"], out["abi_wrr_nnz"] = int(conn.n), int(W_rr.shape[0]), int(W_rr.nnz)
+    return out
+
+
 def c0_run(cpu: bool, steps: int = 1000):
     """The plain-LIF workload in full (16,384 neurons, lognormal out-degree, 200 Poisson inputs at 150 Hz, dt 0.1 ms,
     1,000 steps; synth.plain_lif_graph, the configuration of the committed C0 golden): the plain HIP path (Brain,
@@ -465,7 +537,7 @@ def main():
                     help="pin the B=64 launch shape (fold mode, steps per launch, workgroups per CU); skips tuning")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-optic-training", action="store_true")
-    ap.add_argument("--only", choices=("b64", "spmv", "b1", "optic-training", "packed", "c0"), default=None,
+    ap.add_argument("--only", choices=("b64", "spmv", "b1", "optic-training", "packed", "c0", "abi"), default=None,
                     help="run one part alone (plus --copy-calibration) and print its figures: the short runs the "
                          "kernel traces and PMC counter passes profile")
     ap.add_argument("--only-b64", action="store_true", help="the same as --only b64")
@@ -497,7 +569,7 @@ def main():
         part = {"spmv": lambda: spmv_roofline(conn, retina, op), "b1": lambda: b1_run(conn, retina, lp, op, rollouts[0],
                                                                                         args.b1_frames),
                 "optic-training": optic_training, "packed": lambda: packed_run(args.packed_steps, log),
-                "c0": lambda: c0_run(not args.no_cpu)}[args.only]()
+                "c0": lambda: c0_run(not args.no_cpu), "abi": lambda: abi_run(conn, retina, lp, op)}[args.only]()
         if args.copy_calibration:
             copy_calibration(dev)
         print(json.dumps(part), flush=True)
