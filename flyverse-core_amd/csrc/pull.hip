@@ -1,4 +1,4 @@
-// Deterministic synaptic input for the plain (B, N) layout: no atomics, the reference CPU order.
+// Deterministic synaptic input for the plain (B, N) layout: no atomic adds of values, the reference CPU order.
 //
 // The reference's CPU path (flyverse/brain.py:679-692) adds w*x into every target in ascending presynaptic order
 // after the decay g *= a_s (brain.py:832), skipping silent presynaptic cells; its CUDA event_scatter
@@ -102,53 +102,343 @@ __global__ __launch_bounds__(256) void k_pull_bits(const int* __restrict__ ptr, 
     }
 }
 
-// CSC-compatible deterministic scatter: the contract of launch_events (neural.cu:124) without atomics.  One
-// wavefront per (fly, tile of kTile posts) holds g's tile in LDS and walks the transmitting presynaptic cells in
-// ascending order; a cell's out-edges have distinct posts, so one pass over them needs no ordering, and the walk
-// itself gives every post its contributions in ascending presynaptic order.  Work grows with the number of tiles,
-// so this is the small-graph path; Brain uses the pull above.
-constexpr int kTile = 2048;
+// CSC-compatible deterministic scatter: the contract of launch_events (neural.cu:124) without atomic adds to g.  One
+// 1,024-thread workgroup per (fly, tile of kEvTile posts) holds g's tile in LDS.  It gathers the transmitting
+// presynaptic cells of the walk (x != 0) in ascending order, up to kEvRound a round.  Each thread finds the in-tile
+// out-edges of its cells of the round by binary searches over their ascending posts, and the round's contributions
+// are numbered in edge order (ascending presynaptic cell, then the cell's own edge order: the CSC order).  They are
+// applied kEvHits at a time: written out in number order (post and product), counted per post, placed by an LDS
+// counting sort (the order within a post is whatever the atomics gave), each post's list sorted back into number
+// order, and folded onto the tile in that order.  Rounds and parts go in number order, so every post receives its
+// contributions in ascending presynaptic order, each product rounded first: the reference CPU order
+// (flyverse/brain.py:679-692), bit for bit.
+constexpr int kEvThreads = 1024;
+constexpr int kEvWaves = kEvThreads / kWave;
+constexpr int kEvTile = 4096;                     // posts per workgroup
+constexpr int kEvPer = 4;                         // transmitting cells per thread per round
+constexpr int kEvRound = kEvPer * kEvThreads;     // transmitting cells held per round
+constexpr int kEvHits = 4096;                     // contributions applied per part
+constexpr int kEvScan = 16;                       // walk entries per thread per block
+constexpr int kEvBlock = kEvScan * kEvThreads;    // entry (u, t) of a block is walk entry blk + u * kEvThreads + t
+constexpr int kEvSmall = 8;                       // a post's list ordered by its own thread up to this length
 
-template <typename W>
-__global__ __launch_bounds__(64) void k_events(const int* __restrict__ ptr, const int* __restrict__ post,
-                                               const W* __restrict__ w, const float* __restrict__ x,
-                                               float* __restrict__ g, const int* __restrict__ pre_ids, int n,
-                                               int count) {
-    __shared__ float tile[kTile];
-    int lane = threadIdx.x;
-    int b = blockIdx.y;
-    int lo = blockIdx.x * kTile, hi = min(lo + kTile, n);
-    float* gb = g + (int64_t)b * n;
-    const float* xb = x + (int64_t)b * n;
-    for (int k = lane; k < hi - lo; k += kWave) tile[k] = gb[lo + k];
+struct EvLds {
+    float gt[kEvTile];
+    int fpre[kEvRound];            // the round's transmitting cells; once searched, the placed contributions' numbers
+    float fx[kEvRound];            // their x
+    int fs[kEvRound];              // first in-tile out-edge
+    int fb[kEvRound];              // number of the first contribution
+    int bin[kEvTile + 1];          // per-post contributions of a part -> list bounds
+    uint16_t hp[kEvHits];          // a part's contributions in number order: post within the tile
+    float hv[kEvHits];             //                                         product
+    float hc[kEvHits];             // placed contributions: product
+    int tab[kEvScan * kEvWaves];   // a block's transmitting entries per (entry column, wavefront) -> their first position
+    int wsum[kEvWaves];
+    int total;
+    int nlong;                     // a part's lists longer than kEvSmall, and their posts
+    int longs[kEvHits / kEvSmall];
+};
+
+// Exclusive prefix of v over the workgroup's threads in thread order, and (in total) the sum; every thread calls it.
+__device__ __forceinline__ int ev_scan(int v, int* wsum, int& total) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int u = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += u;
+    }
+    if (lane == kWave - 1) wsum[wave] = incl;
     __syncthreads();
-    int size = count < 0 ? n : count;
-    for (int base = 0; base < size; base += kWave) {
-        int k = base + lane;
-        int pre = 0;
-        float xv = 0.0f;
-        if (k < size) { pre = count < 0 ? k : pre_ids[k]; xv = xb[pre]; }
-        uint64_t m = __builtin_amdgcn_ballot_w64(xv != 0.0f);
-        while (m) {
-            int j = __builtin_ctzll(m);
-            m &= m - 1;
-            int p = __builtin_amdgcn_readlane(pre, j);
-            float xp = fh_readlane(xv, j);
-            int s = ptr[p], e = ptr[p + 1];
-            for (int q0 = s; q0 < e; q0 += kWave) {
-                int q = q0 + lane;
-                int t = q < e ? post[q] : n;
-                if (t >= lo && t < hi) {
-                    float c = (float)w[q] * xp;
-                    tile[t - lo] = tile[t - lo] + c;
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kEvWaves; ++i) {
+        const int s = wsum[i];
+        base += i < wave ? s : 0;
+        tot += s;
+    }
+    total = tot;
+    __syncthreads();
+    return base + incl - v;
+}
+
+// One round: the nf cells in L.fpre / L.fx (ascending); thread t searches cells t + j * kEvThreads.
+template <typename W>
+__device__ __forceinline__ void ev_round(EvLds& L, const int* __restrict__ ptr, const int* __restrict__ post,
+                                         const W* __restrict__ w, int nf, int lo, int hi) {
+    const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+    // the in-tile edges [s_j, s_j + k_j) of each searched cell: lower bounds of lo and of hi over its ascending
+    // posts, the searches of a thread's cells in lockstep so their loads are in flight together
+    int a[kEvPer], r[kEvPer], e[kEvPer];
+#pragma unroll
+    for (int j = 0; j < kEvPer; ++j) {
+        const int f = t + j * kEvThreads;
+        a[j] = 0;
+        e[j] = 0;
+        if (f < nf) {
+            const int p = L.fpre[f];
+            a[j] = ptr[p];
+            e[j] = ptr[p + 1];
+        }
+        r[j] = e[j];
+    }
+    int sj[kEvPer], kj[kEvPer];
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        const int bound = pass ? hi : lo;
+        for (;;) {
+            bool any = false;
+            int mid[kEvPer], pv[kEvPer];
+#pragma unroll
+            for (int j = 0; j < kEvPer; ++j) {
+                mid[j] = (a[j] + r[j]) >> 1;
+                any = any || a[j] < r[j];
+            }
+            if (!any) break;
+#pragma unroll
+            for (int j = 0; j < kEvPer; ++j) pv[j] = a[j] < r[j] ? post[mid[j]] : 0;
+#pragma unroll
+            for (int j = 0; j < kEvPer; ++j) {
+                if (a[j] < r[j]) {
+                    if (pv[j] < bound) a[j] = mid[j] + 1;
+                    else r[j] = mid[j];
                 }
-                // posts are ascending within the column: stop once this round has passed the tile
-                if (__builtin_amdgcn_readlane(t, kWave - 1) >= hi) break;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kEvPer; ++j) {
+            if (pass == 0) {
+                sj[j] = a[j];
+                r[j] = e[j];
+            } else {
+                kj[j] = a[j] - sj[j];
             }
         }
     }
+    // numbers: cell f's contributions are fb[f] .. fb[f] + k - 1 (fb ascends with f)
+    int running = 0;
+#pragma unroll
+    for (int j = 0; j < kEvPer; ++j) {
+        int sum;
+        const int base = running + ev_scan(kj[j], L.wsum, sum);
+        running += sum;
+        const int f = t + j * kEvThreads;
+        if (f < nf) {
+            L.fs[f] = sj[j];
+            L.fb[f] = base;
+        }
+    }
+    const int H = running;
+    const int span = hi - lo;
+    int* he = L.fpre;                                  // the cells are searched: their slots take the placed numbers
     __syncthreads();
-    for (int k = lane; k < hi - lo; k += kWave) gb[lo + k] = tile[k];
+    for (int h0 = 0; h0 < H; h0 += kEvHits) {
+        const int m = min(kEvHits, H - h0);
+        // the part's contributions in number order: the cell of number h is the last with fb <= h
+        for (int i = t; i < m; i += kEvThreads) {
+            const int h = h0 + i;
+            int c0 = 0, c1 = nf;                       // fb[c0] <= h < fb[c1] (fb[nf] taken as H)
+            while (c1 - c0 > 1) {
+                const int cm = (c0 + c1) >> 1;
+                if (L.fb[cm] <= h) c0 = cm;
+                else c1 = cm;
+            }
+            const int edge = L.fs[c0] + (h - L.fb[c0]);
+            L.hp[i] = (uint16_t)(post[edge] - lo);
+            L.hv[i] = (float)w[edge] * L.fx[c0];
+        }
+        for (int i = t; i <= kEvTile; i += kEvThreads) L.bin[i] = 0;
+        __syncthreads();
+        for (int i = t; i < m; i += kEvThreads) atomicAdd(&L.bin[L.hp[i]], 1);
+        __syncthreads();
+        // list ends (inclusive prefix of the counts): kEvTile / kEvThreads posts per thread
+        constexpr int kPosts = kEvTile / kEvThreads;
+        int cnt[kPosts], own = 0;
+#pragma unroll
+        for (int u = 0; u < kPosts; ++u) {
+            cnt[u] = L.bin[t * kPosts + u];
+            own += cnt[u];
+        }
+        int total;
+        int end = ev_scan(own, L.wsum, total);
+#pragma unroll
+        for (int u = 0; u < kPosts; ++u) {
+            end += cnt[u];
+            L.bin[t * kPosts + u] = end;
+        }
+        if (t == 0) L.bin[kEvTile] = total;
+        __syncthreads();
+        // placement from the list ends down: afterwards bin[p] is the start of post p's list and bin[p + 1] its end
+        for (int i = t; i < m; i += kEvThreads) {
+            const int slot = atomicSub(&L.bin[L.hp[i]], 1) - 1;
+            he[slot] = i;
+            L.hc[slot] = L.hv[i];
+        }
+        __syncthreads();
+        // each post's contributions folded onto the tile in number order: a list of at most kEvSmall by its post's
+        // thread, in registers; a longer one (a post of high in-degree) by a wavefront, each entry moved to its rank
+        // among the list's numbers (hp / hv, spent, take the list in number order); a part has fewer than
+        // kEvHits / kEvSmall long lists
+        if (t == 0) L.nlong = 0;
+        __syncthreads();
+        for (int pt = t; pt < span; pt += kEvThreads) {
+            const int s0 = L.bin[pt], k = L.bin[pt + 1] - s0;
+            if (k == 0) continue;
+            if (k > kEvSmall) {
+                L.longs[atomicAdd(&L.nlong, 1)] = pt;
+                continue;
+            }
+            // the list in registers; each entry's rank among the numbers, then the adds in rank order
+            int key[kEvSmall];
+            float val[kEvSmall];
+#pragma unroll
+            for (int q = 0; q < kEvSmall; ++q) {
+                key[q] = q < k ? he[s0 + q] : INT_MAX;
+                val[q] = q < k ? L.hc[s0 + q] : 0.0f;
+            }
+            int rk[kEvSmall];
+#pragma unroll
+            for (int i = 0; i < kEvSmall; ++i) {
+                rk[i] = 0;
+#pragma unroll
+                for (int j = 0; j < kEvSmall; ++j) rk[i] += key[j] < key[i];
+            }
+            float acc = L.gt[pt];
+#pragma unroll
+            for (int r = 0; r < kEvSmall; ++r) {
+                if (r < k) {
+                    float v = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < kEvSmall; ++i) v = rk[i] == r ? val[i] : v;
+                    acc = acc + v;
+                }
+            }
+            L.gt[pt] = acc;
+        }
+        __syncthreads();
+        const int nlong = L.nlong;
+        for (int li = wave; li < nlong; li += kEvWaves) {
+            const int pt = L.longs[li];
+            const int s0 = L.bin[pt], s1 = L.bin[pt + 1];
+            for (int s = s0 + lane; s < s1; s += kWave) {
+                const int key = he[s];
+                int rank = 0;
+                int q = s0;
+                for (; q + 8 <= s1; q += 8) {          // eight reads in flight
+                    int v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = he[q + u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) rank += v[u] < key;
+                }
+                for (; q < s1; ++q) rank += he[q] < key;
+                L.hp[s0 + rank] = (uint16_t)key;
+                L.hv[s0 + rank] = L.hc[s];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == 0) {
+                float acc = L.gt[pt];
+                int s = s0;
+                for (; s + 8 <= s1; s += 8) {          // eight reads in flight, then their adds in order
+                    float v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) v[q] = L.hv[s + q];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc = acc + v[q];
+                }
+                for (; s < s1; ++s) acc = acc + L.hv[s];
+                L.gt[pt] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename W>
+__global__ __launch_bounds__(kEvThreads) void k_events(const int* __restrict__ ptr, const int* __restrict__ post,
+                                                       const W* __restrict__ w, const float* __restrict__ x,
+                                                       float* __restrict__ g, const int* __restrict__ pre_ids, int n,
+                                                       int count) {
+    extern __shared__ __align__(16) unsigned char ev_lds[];
+    EvLds& L = *reinterpret_cast<EvLds*>(ev_lds);
+    const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+    const int lo = blockIdx.x * kEvTile, hi = min(lo + kEvTile, n);
+    float* gb = g + (int64_t)blockIdx.y * n;
+    const float* xb = x + (int64_t)blockIdx.y * n;
+    for (int i = t; i < hi - lo; i += kEvThreads) L.gt[i] = gb[lo + i];
+    const int size = count < 0 ? n : count;
+    int nf = 0;                                    // cells held for the round (the same in every thread)
+    for (int blk = 0; blk < size; blk += kEvBlock) {
+        // entry (u, t) = walk entry blk + u * kEvThreads + t: each wavefront's loads of one column are contiguous.
+        // The block's transmitting entries in walk order get positions: column-major over (column, wavefront),
+        // then lane order within the wavefront.
+        auto entry = [&](int u, int& pre) -> float {
+            const int k = blk + u * kEvThreads + t;
+            if (k >= size) return 0.0f;
+            pre = count < 0 ? k : pre_ids[k];
+            return xb[pre];
+        };
+#pragma unroll
+        for (int u = 0; u < kEvScan; ++u) {
+            int pre;
+            const uint64_t bal = __builtin_amdgcn_ballot_w64(entry(u, pre) != 0.0f);
+            if (lane == 0) L.tab[u * kEvWaves + wave] = __builtin_popcountll(bal);
+        }
+        __syncthreads();
+        if (wave == 0) {                           // exclusive prefix of the kEvScan * kEvWaves counts
+            constexpr int kPer = kEvScan * kEvWaves / kWave;
+            int c[kPer], own = 0;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                c[q] = L.tab[lane * kPer + q];
+                own += c[q];
+            }
+            int incl = own;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int v = __shfl_up(incl, d, kWave);
+                if (lane >= d) incl += v;
+            }
+            int run = incl - own;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                L.tab[lane * kPer + q] = run;
+                run += c[q];
+            }
+            if (lane == kWave - 1) L.total = incl;
+        }
+        __syncthreads();
+        const int total = L.total;
+        for (int done = 0; done < total;) {
+            const int take = min(total - done, kEvRound - nf);   // block positions [done, done + take) go next
+#pragma unroll
+            for (int u = 0; u < kEvScan; ++u) {
+                int pre = 0;
+                const float xv = entry(u, pre);
+                const bool fired = xv != 0.0f;
+                const uint64_t bal = __builtin_amdgcn_ballot_w64(fired);
+                const int q = L.tab[u * kEvWaves + wave] +
+                              (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (fired && q >= done && q < done + take) {
+                    L.fpre[nf + q - done] = pre;
+                    L.fx[nf + q - done] = xv;
+                }
+            }
+            nf += take;
+            done += take;
+            __syncthreads();
+            if (nf == kEvRound) {
+                ev_round(L, ptr, post, w, nf, lo, hi);
+                nf = 0;
+            }
+        }
+    }
+    if (nf) ev_round(L, ptr, post, w, nf, lo, hi);
+    __syncthreads();
+    for (int i = t; i < hi - lo; i += kEvThreads) gb[lo + i] = L.gt[i];
 }
 
 // The synaptic input of the clocked step (flyverse/brain.py:903-921): per (fly, row) wavefront, g = g * a_s[row]
@@ -250,13 +540,20 @@ FH_EXPORT int launch_pull_bits(void** p, int n, int n_full, int batch, float a_s
 FH_EXPORT int launch_events(void** p, int n, int batch, int half, int count, void* stream) {
     if (n < 0 || batch < 0) return FH_ERR_ARGS;
     if ((int64_t)n * batch == 0) return 0;
-    dim3 grid((unsigned)((n + kTile - 1) / kTile), (unsigned)batch);
+    if (batch > 65535) return FH_ERR_ARGS;
+    const dim3 grid((unsigned)((n + kEvTile - 1) / kEvTile), (unsigned)batch);
+    const size_t lds = sizeof(EvLds);
+    hipError_t e = hipFuncSetAttribute(half ? (const void*)k_events<__half> : (const void*)k_events<float>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return (int)e;
     if (half)
-        hipLaunchKernelGGL(k_events<__half>, grid, dim3(64), 0, (hipStream_t)stream, (const int*)p[0], (const int*)p[1],
-                           (const __half*)p[2], (const float*)p[3], (float*)p[4], (const int*)p[5], n, count);
+        hipLaunchKernelGGL(k_events<__half>, grid, dim3(kEvThreads), lds, (hipStream_t)stream, (const int*)p[0],
+                           (const int*)p[1], (const __half*)p[2], (const float*)p[3], (float*)p[4], (const int*)p[5], n,
+                           count);
     else
-        hipLaunchKernelGGL(k_events<float>, grid, dim3(64), 0, (hipStream_t)stream, (const int*)p[0], (const int*)p[1],
-                           (const float*)p[2], (const float*)p[3], (float*)p[4], (const int*)p[5], n, count);
+        hipLaunchKernelGGL(k_events<float>, grid, dim3(kEvThreads), lds, (hipStream_t)stream, (const int*)p[0],
+                           (const int*)p[1], (const float*)p[2], (const float*)p[3], (float*)p[4], (const int*)p[5], n,
+                           count);
     FH_LAUNCH_CHECK();
 }
 

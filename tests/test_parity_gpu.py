@@ -159,6 +159,28 @@ def test_pull_and_events_bit_exact(B, density, activity):
         assert_bits(N_(g), ref, f"events compact={compact}")
 
 
+@pytest.mark.parametrize("activity", [0.02, 0.6])
+def test_events_over_several_tiles_and_rounds(activity):
+    """launch_events on a graph of several post tiles (10,000 cells, tiles of 4,096 posts, the last one partial), at
+    an activity that needs one round of transmitting cells and at one that needs two (more than 4,096 transmitting
+    cells), with fractional and negative x, posts of high in-degree (whose contributions in a round span several
+    placement parts), the full and the compact walk: bit-exact against the oracle's ascending-presynaptic order."""
+    rng = np.random.default_rng(17)
+    n, B = 10000, 2
+    W = random_csr(rng, n, n, 0.002, long_rows=(5, 4096, 9999), long_len=6000)
+    x = (rng.random((B, n)) < activity).astype(np.float32)
+    x[:, ::3] *= rng.uniform(-1.0, 1.0, (B, len(range(0, n, 3)))).astype(np.float32)
+    g0 = rng.normal(0, 4, (B, n)).astype(np.float32)
+    Wc = _csc(W)
+    ref = g0.copy()
+    oracle.events(Wc.indptr, Wc.indices, Wc.data, x, ref)
+    for compact in (False, True):
+        g = T(g0)
+        pre_ids = T(np.flatnonzero(np.diff(Wc.indptr)).astype(np.int32)) if compact else None
+        hip.event_scatter(T(Wc.indptr.astype(np.int32)), T(Wc.indices.astype(np.int32)), T(Wc.data), T(x), g, pre_ids)
+        assert_bits(N_(g), ref, f"events compact={compact}")
+
+
 def test_pull_half_weights():
     rng = np.random.default_rng(2)
     n, B = 600, 2
