@@ -314,9 +314,22 @@ __global__ __launch_bounds__(128) void k_means(const int* __restrict__ ptr, cons
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= groups * batch) return;
     int b = t / groups, k = t % groups;
+    const float* rb = rates + (int64_t)b * n;
     double sum = 0.0;
-    for (int j = ptr[k]; j < ptr[k + 1]; ++j) sum += (double)rates[(int64_t)b * n + idx[j]];
-    int cnt = ptr[k + 1] - ptr[k];
+    int j = ptr[k];
+    const int e = ptr[k + 1];
+    for (; j + 8 <= e; j += 8) {                       // eight gathers in flight, then their adds in order
+        int c[8];
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = idx[j + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = rb[c[q]];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sum += (double)v[q];
+    }
+    for (; j < e; ++j) sum += (double)rb[idx[j]];
+    int cnt = e - ptr[k];
     out[(int64_t)b * stride + k] = cnt ? sum / cnt : 0.0;
 }
 
