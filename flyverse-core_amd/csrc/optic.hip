@@ -374,8 +374,8 @@ FH_EXPORT int launch_optic_substep(void** p, int n_rate, int ngroups, int batch,
 }
 
 // The substep with its W_rr product by panels: p = [W_rr blk_row (nblk+1; blocks of launch_spmv_panel), chunk_ptr,
-// pidx, pw, dr_cur, dr_next, v, adapt, pr, spk (B, n_rate each), a, b (n_rate)].  Same values as
-// launch_optic_substep.
+// pidx, pw (slot order), dr_cur, dr_next, v, adapt, pr, spk (B, n_rate each), a, b (n_rate), nat (slot -> chunk, or
+// null)].  Same values as launch_optic_substep.
 FH_EXPORT int launch_optic_substep_panel(void** p, int n_rate, int nblk, int batch, float gr, float ga, float aad,
                                          void* stream) {
     if (n_rate < 0 || nblk < 0 || batch < 0 || batch > 65535) return FH_ERR_ARGS;
@@ -386,7 +386,7 @@ FH_EXPORT int launch_optic_substep_panel(void** p, int n_rate, int nblk, int bat
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds_floats * 4);
     if (e != hipSuccess) return (int)e;
     SubstepArgs<fh_panel::Panel> s{fh_panel::Panel{(const int*)p[0], (const int*)p[1], (const int4*)p[2],
-                                                   (const float4*)p[3], n_rate, pc},
+                                                   (const float4*)p[3], (const int*)p[12], n_rate, pc},
                                    (const float*)p[4], (float*)p[5], (float*)p[6], (float*)p[7], (const float*)p[8],
                                    (const float*)p[9], (const float*)p[10], (const float*)p[11], n_rate, gr, ga, aad};
     hipLaunchKernelGGL(k_optic_substep_panel, dim3(nblk, batch), dim3(fh_panel::kThreads), (size_t)lds_floats * 4,

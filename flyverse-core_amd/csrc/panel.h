@@ -1,9 +1,11 @@
 // Panel product of one fly's row block, shared by the B = 1 product kernel and the fused optic substep.
 //
 // Storage: the chunk-padded CSR of groups.h (chunk c = entries [16 c, 16 c + 16) of pidx / pw, pads idx -1, entries
-// of a row in ascending column order), cut on the host into blocks of consecutive rows of at most kPanelChunks
-// chunks per thread.  One 1024-thread workgroup works on one (block, fly) for the whole launch: thread t holds the
-// indices and weights of the block's chunks t, t + 1024, t + 2048 in registers.  x is staged in LDS one panel of
+// of a row in ascending column order), cut on the host into blocks of consecutive rows of at most kChunks chunks
+// per thread; within a block the chunks may sit in any slot order (the host orders them by first column, so the
+// chunks of a wavefront cover nearby columns), nat giving each slot's chunk.  One 1024-thread workgroup works on
+// one (block, fly) for the whole launch: thread t holds the indices and weights of the block's slots t, t + 1024,
+// t + 2048 in registers.  x is staged in LDS one panel of
 // columns at a time; in each pass a thread adds, in entry order, the products of its chunks' entries whose column
 // lies in the panel.  The entries of a chunk ascend in column and the panels are visited in ascending order, so
 // each chunk is still one sequential fold of its products from +0, in entry order.  The chunk sums then go through
@@ -41,8 +43,9 @@ inline int lds_floats(int pc) { return max(pc, kChunks * kThreads); }
 struct Panel {
     const int* blk_row;    // first row of each block, nblk + 1
     const int* chunk_ptr;  // rows + 1
-    const int4* pidx;
+    const int4* pidx;      // the chunk-padded storage, each block's chunks in slot order
     const float4* pw;
+    const int* nat;        // slot -> chunk in the natural order (where its sum goes); null: the same
     int cols, panel;       // columns of x, floats per panel
 };
 
@@ -89,9 +92,11 @@ __device__ __forceinline__ void block_product(const Panel& m, int blk, const flo
     // largest)
     uint32_t iv[kChunks][16];
     float wv[kChunks][16];
+    int nat[kChunks];                                  // where each held chunk's sum goes (-1: no chunk)
 #pragma unroll
     for (int u = 0; u < kChunks; ++u) {
         const int c = cb + t + u * kThreads;
+        nat[u] = c < ce ? (m.nat ? m.nat[c] : c) : -1;
         if (c < ce) {
             const int4* ip = m.pidx + (int64_t)c * 4;
             const float4* wp = m.pw + (int64_t)c * 4;
@@ -154,7 +159,8 @@ __device__ __forceinline__ void block_product(const Panel& m, int blk, const flo
     }
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < kChunks; ++u) xs[t + u * kThreads] = s[u];     // chunk cb + i's sum at xs[i]
+    for (int u = 0; u < kChunks; ++u)
+        if (nat[u] >= 0) xs[nat[u] - cb] = s[u];       // chunk cb + i's sum at xs[i]
     __syncthreads();
     for (int row = r0 + t; row < r1; row += kThreads) {
         const int ca = m.chunk_ptr[row] - cb, cz = m.chunk_ptr[row + 1] - cb;

@@ -355,7 +355,8 @@ FH_EXPORT int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batc
 
 // Panel product, same order as launch_csr, over the chunk-padded storage of launch_spmv_grouped: p = [blk_row
 // (nblk+1): first row of each block (at most 3 * 1024 chunks), chunk_ptr (rows+1), pidx (16*nchunks), pw
-// (16*nchunks), x (B, cols), y (B, rows)]; panel = columns of x staged at a time (0: as many as fit, balanced).
+// (16*nchunks), x (B, cols), y (B, rows), nat (nchunks: the chunk each slot of pidx / pw holds, a permutation within
+// each block; null for the natural order)]; panel = columns of x staged at a time (0: as many as fit, balanced).
 // y = scale * W x (+ y if accumulate).
 FH_EXPORT int launch_spmv_panel(void** p, int rows, int cols, int nblk, int batch, int panel, float scale,
                                 int accumulate, void* stream) {
@@ -366,7 +367,8 @@ FH_EXPORT int launch_spmv_panel(void** p, int rows, int cols, int nblk, int batc
     hipError_t e = hipFuncSetAttribute((const void*)k_spmv_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        lds_floats * 4);
     if (e != hipSuccess) return (int)e;
-    const fh_panel::Panel m{(const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3], cols, pc};
+    const fh_panel::Panel m{(const int*)p[0], (const int*)p[1], (const int4*)p[2], (const float4*)p[3], (const int*)p[6],
+                              cols, pc};
     hipLaunchKernelGGL(k_spmv_panel, dim3((unsigned)nblk, (unsigned)batch), dim3(fh_panel::kThreads),
                        (size_t)lds_floats * 4, (hipStream_t)stream, m, (const float*)p[4], (float*)p[5], rows, scale,
                        accumulate);

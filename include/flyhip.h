@@ -114,7 +114,8 @@ int launch_spmv_lds(void** p, int rows, int cols, int ntiles, int batch, int ran
  * chunks (at most 3 per thread) in registers for the whole launch, and x is staged in LDS one panel of columns at a
  * time, each workgroup reading x once; same additions in the same order as launch_csr.  p = [blk_row (nblk+1): first
  * row of each block of at most 3 * 1024 chunks, chunk_ptr (rows+1), pidx (16*nchunks), pw (16*nchunks), x (batch,
- * cols), y (batch, rows)]; panel = columns staged at a time (0: as few equal panels as the LDS allows).
+ * cols), y (batch, rows), nat (nchunks: the chunk each slot of pidx / pw holds, a permutation within each block, or
+ * null)]; panel = columns staged at a time (0: as few equal panels as the LDS allows).
  * y = scale * W x (+ y if accumulate).  Replaces the _mv products of flyverse/optic.py:161-165. */
 int launch_spmv_panel(void** p, int rows, int cols, int nblk, int batch, int panel, float scale, int accumulate,
                       void* stream);
@@ -140,8 +141,8 @@ int launch_optic_substep(void** p, int n_rate, int ngroups, int batch, float gr,
 
 /* launch_optic_substep with the W_rr product by persistent panels (launch_spmv_panel's blocks and order, which are
  * launch_csr's additions): p = [W_rr blk_row (nblk+1), chunk_ptr, pidx, pw, dr_cur, dr_next, v, adapt, pr, spk
- * (batch, n_rate each), a, b (n_rate)].  Same values as launch_optic_substep.  Replaces a launch_csr + launch_optic
- * pair (flyverse/optic.py:474-483 then 525-551). */
+ * (batch, n_rate each), a, b (n_rate), nat (as launch_spmv_panel's)].  Same values as launch_optic_substep.
+ * Replaces a launch_csr + launch_optic pair (flyverse/optic.py:474-483 then 525-551). */
 int launch_optic_substep_panel(void** p, int n_rate, int nblk, int batch, float gr, float ga, float aad,
                                void* stream);
 
